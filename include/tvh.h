@@ -1,0 +1,227 @@
+/*
+ * tvh.h - C ABI of the MI355X (gfx950) table-commitment library.
+ *
+ * Drop-in for the reference prover's table-commitment hot path: Goldilocks
+ * coset low-degree extension, Tip5 row hashing and Tip5 Merkle trees.  Every
+ * entry point is `extern "C"`, takes plain pointers and sizes, returns an int
+ * status and never throws.  Field elements are canonical u64 in [0, p),
+ * p = 2^64 - 2^32 + 1 (the reference's BFieldElement,
+ * include/types/b_field_element.hpp:31-41); `_dev` entry points require
+ * canonical inputs and produce canonical outputs.
+ *
+ * Layouts (the reference GPU layouts):
+ *   tables   column-major, element (row r, column c) at c*num_rows + r
+ *            (include/gpu/kernels/ntt_kernel.cuh:79-84)
+ *   XFE      limb-major columns, limb k of XFE column c at (3c + k)*num_rows + r
+ *            (src/gpu/kernels/row_hash_kernel.cu:442-473)
+ *   digests  5 contiguous u64 per row
+ *   trees    TVH_TREE_LEVEL_ORDER: leaves [0,n), then each level, root at node
+ *            2n-2 (src/gpu/kernels/merkle_kernel.cu:60-102);
+ *            TVH_TREE_HEAP: node i at i (1-indexed), leaves [n,2n), root at node 1
+ *            (src/merkle/merkle_tree.cpp:12-33).  Node j occupies u64 [5j, 5j+5).
+ *
+ * `_dev` calls take device pointers and a hipStream_t passed as void*
+ * (NULL = the default stream); they are asynchronous on that stream, never
+ * synchronise it and allocate no device memory, except where stated
+ * (tvh_ntt_*_batched_dev and tvh_merkle_root_dev use a per-device arena that
+ * grows on first use of a larger size).  `_host` calls take host pointers and
+ * are synchronous.
+ */
+#ifndef TVH_H
+#define TVH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define TVH_OK 0
+#define TVH_ERR_INVALID_ARG 1   /* null pointer, zero/oversized dimension      */
+#define TVH_ERR_NOT_POW2 2      /* length must be a power of two               */
+#define TVH_ERR_OOM 3           /* device allocation failed                    */
+#define TVH_ERR_DEVICE 4        /* HIP runtime or kernel launch error          */
+#define TVH_ERR_LAYOUT 5        /* unknown tree layout                         */
+#define TVH_ERR_NOT_CANONICAL 6 /* host input value >= p                       */
+#define TVH_ERR_UNSUPPORTED 7   /* shape outside the supported range           */
+#define TVH_ERR_RANGE 8         /* index out of range                          */
+
+#define TVH_TREE_LEVEL_ORDER 0
+#define TVH_TREE_HEAP 1
+
+#define TVH_P 0xFFFFFFFF00000001ull
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Set the current HIP device and build its constant tables (twiddles).
+ * Thread-safe; idempotent.  Replaces the implicit ntt_init_constants()
+ * (include/gpu/kernels/ntt_kernel.cuh:19-24). */
+int tvh_init(int device);
+
+/* Human-readable name of a status code (static string). */
+const char* tvh_status_string(int status);
+
+/* Message of the last failing call on this thread (static per-thread buffer). */
+const char* tvh_last_error(void);
+
+/* ---- NTT (src/gpu/kernels/ntt_kernel.cu; include/gpu/kernels/ntt_kernel.cuh) */
+
+/* Forward NTT of num_cols columns of length n (power of two, n <= 2^24),
+ * column-major, in place, natural order in and out:
+ * out[j] = sum_i in[i] * w_n^(i*j), w_n = 7^((p-1)/n).
+ * Replaces ntt_forward_batched_gpu (ntt_kernel.cuh:88-93). */
+int tvh_ntt_forward_batched_dev(uint64_t* d_data, size_t n, size_t num_cols, void* stream);
+
+/* Inverse NTT including the factor n^-1.
+ * Replaces ntt_inverse_batched_gpu (ntt_kernel.cuh:102-107). */
+int tvh_ntt_inverse_batched_dev(uint64_t* d_data, size_t n, size_t num_cols, void* stream);
+
+/* Single-column forms. Replace ntt_forward_gpu / ntt_inverse_gpu
+ * (ntt_kernel.cuh:37-58). */
+int tvh_ntt_forward_dev(uint64_t* d_data, size_t n, void* stream);
+int tvh_ntt_inverse_dev(uint64_t* d_data, size_t n, void* stream);
+
+/* ---- low-degree extension (src/gpu/kernels/randomized_lde_kernel.cu) ------ */
+
+/* Randomized coset LDE of num_cols B-field columns:
+ *   poly_c = INTT(trace_c) scaled by trace_offset^-i, plus
+ *            (x^n - trace_offset^n) * r_c(x), r_c = d_rand[c*rand_len + i]
+ *   out_c[j] = poly_c(target_offset * w_m^j), j < target_len
+ * trace_len and target_len are powers of two, target_len = E * trace_len
+ * with E in {1,2,4,8,16}; rand_len <= trace_len; d_rand may be NULL when
+ * rand_len == 0.  d_trace is column-major trace_len x num_cols, d_out is
+ * column-major target_len x num_cols; they must not overlap.  d_out is the
+ * only working memory (no scratch).
+ * Replaces randomized_lde_batch_gpu (randomized_lde_kernel.cuh:62-95) and
+ * randomized_lde_batch_gpu_preallocated (:103-117). */
+int tvh_lde_batch_dev(const uint64_t* d_trace, size_t num_cols, size_t trace_len,
+                      const uint64_t* d_rand, size_t rand_len, uint64_t trace_offset,
+                      uint64_t target_offset, size_t target_len, uint64_t* d_out,
+                      void* stream);
+
+/* One column.  Replaces randomized_lde_column_gpu (randomized_lde_kernel.cuh:34-60). */
+int tvh_lde_column_dev(const uint64_t* d_trace, size_t trace_len, const uint64_t* d_rand,
+                       size_t rand_len, uint64_t trace_offset, uint64_t target_offset,
+                       size_t target_len, uint64_t* d_out, void* stream);
+
+/* XFE columns in the interleaved layout of randomized_xfe_lde_batch_gpu
+ * (randomized_lde_kernel.cuh:184-199): limb k of (col, row) at
+ * (col*trace_len + row)*3 + k, output likewise with target_len.  Randomizers
+ * are BFE (lifted to limb 0) or, with use_xfe_randomizers, 3 limbs each at
+ * (col*rand_len + i)*3 + k.  Evaluation truncates coefficients beyond
+ * target_len (src/lde/lde_randomized.cpp:270-275).  Uses d_xfe_output's
+ * memory plus a per-device arena for the limb-major staging copy. */
+int tvh_lde_xfe_batch_dev(const uint64_t* d_xfe_trace, size_t num_cols, size_t trace_len,
+                          const uint64_t* d_rand, size_t rand_len, uint64_t trace_offset,
+                          uint64_t target_offset, size_t target_len, uint64_t* d_xfe_out,
+                          int use_xfe_randomizers, void* stream);
+
+/* Limb-major XFE columns (the aux-table commit layout, (3c + k)*n + r): the
+ * same LDE on 3*num_xfe_cols B-field columns, with the XFE truncation rule and
+ * randomizer lifting of tvh_lde_xfe_batch_dev.  No scratch. */
+int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t trace_len,
+                          const uint64_t* d_rand, size_t rand_len, uint64_t trace_offset,
+                          uint64_t target_offset, size_t target_len, uint64_t* d_out,
+                          int use_xfe_randomizers, void* stream);
+
+/* ---- Tip5 (src/gpu/kernels/row_hash_kernel.cu, tip5_kernel.cu) ------------- */
+
+/* digests[r] = Tip5::hash_varlen(row r) over a column-major table.
+ * Replaces hash_bfield_rows_gpu (row_hash_kernel.cuh:27-33). */
+int tvh_hash_bfield_rows_dev(const uint64_t* d_table, size_t num_rows, size_t num_cols,
+                             uint64_t* d_digests, void* stream);
+
+/* XFE rows in limb-major layout, flattened (c0,c1,c2) per XFE column.
+ * Replaces hash_xfield_rows_gpu (row_hash_kernel.cuh:44-50). */
+int tvh_hash_xfield_rows_dev(const uint64_t* d_table, size_t num_rows, size_t num_xfe_cols,
+                             uint64_t* d_digests, void* stream);
+
+/* In-place Tip5 permutation of num_states 16-element states.
+ * Replaces Backend::tip5_permutation_batch (include/backend/backend.hpp:95-101). */
+int tvh_tip5_permutation_batch_dev(uint64_t* d_states, size_t num_states, void* stream);
+
+/* out[i] = hash_pair(left[i], right[i]).  Replaces Backend::hash_pairs
+ * (include/backend/backend.hpp:103-115) and hash_pairs_gpu (tip5_kernel.cu:210-248). */
+int tvh_hash_pairs_dev(const uint64_t* d_left, const uint64_t* d_right, uint64_t* d_out,
+                       size_t count, void* stream);
+
+/* ---- Merkle (src/gpu/kernels/merkle_kernel.cu; src/merkle/merkle_tree.cpp) -- */
+
+/* Full tree over num_leaves (power of two) digests in the given layout.
+ * d_tree holds (2n-1)*5 u64 (level order) or 2n*5 u64 (heap, node 0 unused).
+ * d_leaves may alias the tree's leaf level (in-place build).
+ * Replaces merkle_tree_gpu (merkle_kernel.cuh:26-38). */
+int tvh_merkle_tree_dev(const uint64_t* d_leaves, uint64_t* d_tree, size_t num_leaves,
+                        int layout, void* stream);
+
+/* Root only (5 u64).  Replaces merkle_root_gpu (merkle_kernel.cuh:40-52). */
+int tvh_merkle_root_dev(const uint64_t* d_leaves, uint64_t* d_root, size_t num_leaves,
+                        void* stream);
+
+/* d_out[q] = tree node with heap index d_heap_idx[q] (1 <= idx < 2n), for a
+ * tree in either layout: the digest gather behind authentication paths and
+ * structures (gpu_stark.cu:1266-1286; gather_kernel.cuh:30-50). */
+int tvh_merkle_gather_dev(const uint64_t* d_tree, size_t num_leaves, int layout,
+                          const uint64_t* d_heap_idx, size_t count, uint64_t* d_out,
+                          void* stream);
+
+/* Host helpers: heap indices of MerkleTree::authentication_path (bottom-up
+ * siblings, merkle_tree.cpp:93-113) and of authentication_structure (needed
+ * minus computable, sorted descending, merkle_tree.cpp:228-265).
+ * *out_count receives the number of indices; out may be NULL to query it. */
+int tvh_auth_path_indices(size_t num_leaves, size_t leaf_index, uint64_t* out, size_t cap,
+                          size_t* out_count);
+int tvh_auth_structure_indices(size_t num_leaves, const size_t* leaf_indices, size_t count,
+                               uint64_t* out, size_t cap, size_t* out_count);
+
+/* ---- whole commit (gpu_stark.cu:2132-2610 step_main_table_commitment) ----- */
+
+/* trace -> codewords (LDE) -> row digests (written into the tree's leaf
+ * level) -> Merkle tree.  d_codewords: target_len*num_cols (column-major);
+ * d_tree: tree buffer of the chosen layout; the root is node 2m-2 (level
+ * order) or node 1 (heap).  Inputs as tvh_lde_batch_dev. */
+int tvh_commit_dev(const uint64_t* d_trace, size_t num_cols, size_t trace_len,
+                   const uint64_t* d_rand, size_t rand_len, uint64_t trace_offset,
+                   uint64_t target_offset, size_t target_len, uint64_t* d_codewords,
+                   uint64_t* d_tree, int layout, void* stream);
+
+/* ---- host-pointer forms (synchronous; CudaBackend-style) ------------------ */
+
+/* Replace CudaBackend::ntt_forward / ntt_inverse (src/gpu/cuda_ntt.cu:13-50). */
+int tvh_ntt_forward_host(uint64_t* data, size_t n);
+int tvh_ntt_inverse_host(uint64_t* data, size_t n);
+
+/* Replace CudaBackend::lde_batch / RandomizedLDE per column
+ * (src/gpu/cuda_lde.cu:26-83; src/lde/lde_randomized.cpp:71-154). */
+int tvh_lde_batch_host(const uint64_t* trace, size_t num_cols, size_t trace_len,
+                       const uint64_t* rand, size_t rand_len, uint64_t trace_offset,
+                       uint64_t target_offset, size_t target_len, uint64_t* out);
+
+/* Replace Backend::tip5_permutation_batch / hash_pairs on host buffers. */
+int tvh_tip5_permutation_batch_host(uint64_t* states, size_t num_states);
+int tvh_hash_pairs_host(const uint64_t* left, const uint64_t* right, uint64_t* out,
+                        size_t count);
+
+/* Row digests of a column-major host table. */
+int tvh_hash_bfield_rows_host(const uint64_t* table, size_t num_rows, size_t num_cols,
+                              uint64_t* digests);
+
+/* Merkle tree of host leaves; tree_out as tvh_merkle_tree_dev (may be NULL),
+ * root_out 5 u64.  Replaces Backend::merkle_root / merkle_tree_full. */
+int tvh_merkle_tree_host(const uint64_t* leaves, size_t num_leaves, int layout,
+                         uint64_t* tree_out, uint64_t* root_out);
+
+/* Whole commit on host buffers (LDETable::compute_lde_and_merkle,
+ * src/lde/lde_table.cpp:86-101).  codewords_out (target_len*num_cols,
+ * column-major) and digests_out (target_len*5) may be NULL. */
+int tvh_commit_host(const uint64_t* trace, size_t num_cols, size_t trace_len,
+                    const uint64_t* rand, size_t rand_len, uint64_t trace_offset,
+                    uint64_t target_offset, size_t target_len, uint64_t* codewords_out,
+                    uint64_t* digests_out, uint64_t* root_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TVH_H */

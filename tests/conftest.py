@@ -43,15 +43,33 @@ def golden():
         return json.load(f)
 
 
-@pytest.fixture(scope="session")
-def tvh():
-    """The HIP library through its C ABI (GPU tests only)."""
-    import tvh_loader
+def tvh_module():
+    """triton-vm-prover_amd/tvh.py (the package directory name is not importable)."""
+    import importlib.util
 
-    lib = tvh_loader.load()
+    if "tvh" in sys.modules:
+        return sys.modules["tvh"]
+    spec = importlib.util.spec_from_file_location(
+        "tvh", os.path.join(ROOT, "triton-vm-prover_amd", "tvh.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["tvh"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def tvhmod():
+    return tvh_module()
+
+
+@pytest.fixture(scope="session")
+def tvh(tvhmod):
+    """The HIP library through its C ABI (GPU tests only).  Fails loudly when the
+    library or the device is missing: there is no fallback."""
+    lib = tvhmod.load()
     rc = lib.tvh_init(0)
     if rc != 0:
-        raise RuntimeError(f"tvh_init(0) failed: {lib.status_string(rc)}")
+        raise RuntimeError(f"tvh_init(0) failed: {lib.status_string(rc)}: {lib.last_error()}")
     return lib
 
 

@@ -1,0 +1,313 @@
+"""HIP path vs the oracle, bit for bit, through the C ABI (include/tvh.h).
+
+Small and medium shapes compare every output with the oracle on the same
+seeded inputs and with the golden fixtures written from the reference's own
+CPU code.  The 2^20 x 379 shape checks sampled columns, rows and tree paths
+against the oracle plus size-independent properties (LDE round trip, paths
+verify to the root).
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from conftest import P
+
+pytestmark = pytest.mark.gpu
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<u8").tobytes()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def T(tvh, tvhmod):
+    import torch
+
+    class Ctx:
+        lib = tvh
+        mod = tvhmod
+
+        @staticmethod
+        def dev(a):
+            return tvhmod.to_device(a)
+
+        @staticmethod
+        def host(t, n=None):
+            h = tvhmod.to_host(t)
+            return h if n is None else h[:n]
+
+        @staticmethod
+        def empty(n):
+            return tvhmod.empty(n)
+
+        @staticmethod
+        def sync():
+            torch.cuda.synchronize()
+
+    return Ctx
+
+
+# ------------------------------------------------------------------ Tip5
+def test_permutation_golden_and_random(T, oracle, golden):
+    states = [c["in"] for c in golden["tip5_permutation"]]
+    outs = [c["out"] for c in golden["tip5_permutation"]]
+    rnd = oracle.splitmix(77, 16 * 1000).reshape(1000, 16)
+    allin = np.concatenate([np.array(states, np.uint64), rnd])
+    d = T.dev(allin.reshape(-1))
+    T.lib.permutation_batch(d, allin.shape[0])
+    got = T.host(d).reshape(-1, 16)
+    assert [[int(x) for x in r] for r in got[: len(outs)]] == outs
+    for i in range(0, 1000, 97):
+        assert np.array_equal(got[len(outs) + i], oracle.permutation(rnd[i]))
+
+
+def test_hash_pairs(T, oracle, golden):
+    for c in golden["hash_pairs"] if "hash_pairs" in golden else golden["hash_pair"]:
+        l, r = oracle.splitmix(c["seed_l"], 5), oracle.splitmix(c["seed_r"], 5)
+        out = T.empty(5)
+        T.lib.hash_pairs(T.dev(l), T.dev(r), out, 1)
+        assert [int(x) for x in T.host(out)] == c["out"]
+    n = 3000
+    l, r = oracle.splitmix(5, 5 * n), oracle.splitmix(6, 5 * n)
+    out = T.empty(5 * n)
+    T.lib.hash_pairs(T.dev(l), T.dev(r), out, n)
+    got = T.host(out).reshape(n, 5)
+    for i in [0, 1, 255, 256, 2999]:
+        assert np.array_equal(got[i], oracle.hash_pair(l[5 * i:5 * i + 5], r[5 * i:5 * i + 5]))
+
+
+@pytest.mark.parametrize("cols", [0, 1, 9, 10, 11, 19, 20, 21, 156, 264, 379])
+def test_hash_rows_widths(T, oracle, cols):
+    rows = 4096
+    t = oracle.splitmix(300 + cols, rows * cols) if cols else np.zeros(0, np.uint64)
+    dg = T.empty(rows * 5)
+    T.lib.hash_bfield_rows(T.dev(t) if cols else T.empty(1), rows, cols, dg)
+    got = T.host(dg).reshape(rows, 5)
+    want = oracle.hash_rows(t, rows, cols)
+    assert np.array_equal(got, want)
+
+
+def test_hash_rows_golden_lengths(T, oracle, golden):
+    for c in golden["hash_varlen"]:
+        if c["len"] == 0:
+            continue
+        x = oracle.splitmix(c["seed"], c["len"])
+        dg = T.empty(5)
+        T.lib.hash_bfield_rows(T.dev(x), 1, c["len"], dg)
+        assert [int(v) for v in T.host(dg)] == c["out"], c["len"]
+
+
+@pytest.mark.parametrize("rows", [1, 63, 257, 1000])
+def test_hash_rows_ragged(T, oracle, rows):
+    cols = 37
+    t = oracle.splitmix(rows, rows * cols)
+    dg = T.empty(rows * 5)
+    T.lib.hash_bfield_rows(T.dev(t), rows, cols, dg)
+    assert np.array_equal(T.host(dg, rows * 5).reshape(rows, 5), oracle.hash_rows(t, rows, cols))
+
+
+def test_hash_xfield_rows(T, oracle):
+    rows, xcols = 512, 88
+    t = oracle.splitmix(44, rows * xcols * 3)
+    dg = T.empty(rows * 5)
+    T.lib.hash_xfield_rows(T.dev(t), rows, xcols, dg)
+    assert np.array_equal(T.host(dg).reshape(rows, 5), oracle.hash_rows(t, rows, 3 * xcols))
+
+
+# ------------------------------------------------------------------ NTT
+@pytest.mark.parametrize("lg", list(range(0, 17)) + [20])
+def test_ntt_batched(T, oracle, lg):
+    n = 1 << lg
+    cols = 3 if lg < 20 else 1
+    x = oracle.splitmix(900 + lg, n * cols)
+    d = T.dev(x)
+    T.lib.ntt_forward_batched(d, n, cols)
+    fwd = T.host(d)
+    for c in range(cols):
+        assert np.array_equal(fwd[c * n:(c + 1) * n], oracle.ntt(x[c * n:(c + 1) * n])), (lg, c)
+    T.lib.ntt_inverse_batched(d, n, cols)
+    assert np.array_equal(T.host(d), x)
+    d2 = T.dev(x)
+    T.lib.ntt_inverse_batched(d2, n, cols)
+    inv = T.host(d2)
+    assert np.array_equal(inv[:n], oracle.intt(x[:n]))
+
+
+def test_ntt_golden(T, oracle, golden):
+    for c in golden["ntt"]:
+        x = oracle.splitmix(c["seed"], c["n"])
+        d = T.dev(x)
+        T.lib.ntt_forward_batched(d, c["n"], 1)
+        assert sha(T.host(d)) == c["fwd_sha256"]
+        d = T.dev(x)
+        T.lib.ntt_inverse_batched(d, c["n"], 1)
+        assert sha(T.host(d)) == c["inv_sha256"]
+
+
+def test_ntt_large_roundtrip(T, oracle):
+    n, cols = 1 << 22, 2
+    x = oracle.splitmix(5, n * cols)
+    d = T.dev(x)
+    T.lib.ntt_forward_batched(d, n, cols)
+    f = T.host(d)
+    # a size-independent pin: the DC term is the column sum
+    for c in range(cols):
+        assert int(f[c * n]) == int(sum(int(v) for v in x[c * n:(c + 1) * n]) % P)
+    T.lib.ntt_inverse_batched(d, n, cols)
+    assert np.array_equal(T.host(d), x)
+
+
+# ------------------------------------------------------------------ LDE
+LDE_CASES = [  # (logn, E, k, trace_off)
+    (0, 4, 0, 1), (1, 4, 0, 1), (2, 4, 0, 1), (3, 2, 0, 1), (4, 4, 3, 1), (5, 1, 0, 1),
+    (5, 1, 4, 3), (6, 8, 7, 1), (7, 16, 0, 1), (8, 4, 98, 1), (9, 4, 5, 5), (10, 4, 0, 1),
+    (11, 2, 0, 1), (12, 4, 0, 1), (13, 4, 16, 1), (14, 4, 0, 1), (15, 8, 0, 1), (16, 4, 0, 1),
+]
+
+
+@pytest.mark.parametrize("logn,E,k,toff", LDE_CASES)
+def test_lde_batch(T, oracle, logn, E, k, toff):
+    n, cols = 1 << logn, 3
+    m = E * n
+    x = oracle.splitmix(1000 + logn, n * cols)
+    r = oracle.splitmix(2000 + k, k * cols) if k else None
+    out = T.empty(m * cols)
+    T.lib.lde_batch(T.dev(x), cols, n, T.dev(r) if k else None, k, toff, 7, m, out)
+    got = T.host(out)
+    want = oracle.lde_table(x, n, cols, m, toff, 7, None if r is None else r.reshape(cols, k))
+    assert np.array_equal(got, want)
+
+
+def test_lde_golden(T, oracle, golden):
+    for c in golden["lde_column"]:
+        if c["m"] // c["n"] > 16 or c["k"] > c["n"]:
+            continue
+        x = oracle.splitmix(c["seed_trace"], c["n"])
+        r = oracle.splitmix(c["seed_rand"], c["k"]) if c["k"] else None
+        out = T.empty(c["m"])
+        T.lib.lde_batch(T.dev(x), 1, c["n"], T.dev(r) if c["k"] else None, c["k"],
+                        c["trace_off"], c["target_off"], c["m"], out)
+        assert [int(v) for v in T.host(out, c["m"])] == c["out"], c
+
+
+def test_lde_xfe_golden(T, oracle, golden):
+    for c in golden["lde_xfe_column"]:
+        n, m, k = c["n"], c["m"], c["k"]
+        x = oracle.splitmix(c["seed_trace"], 3 * n)           # interleaved (n, 3)
+        r = oracle.splitmix(c["seed_rand"], 3 * k if c["xfe_rand"] else k) if k else None
+        out = T.empty(3 * m)
+        T.lib.lde_xfe_batch(T.dev(x), 1, n, T.dev(r) if k else None, k, c["trace_off"],
+                            c["target_off"], m, out, c["xfe_rand"])
+        assert [int(v) for v in T.host(out, 3 * m)] == c["out"], c
+
+
+def test_lde_xfe_limbs_matches_interleaved(T, oracle):
+    n, m, xc, k = 256, 1024, 5, 4
+    x = oracle.splitmix(8, 3 * n * xc)                       # interleaved
+    r = oracle.splitmix(9, 3 * k * xc)
+    a = T.empty(3 * m * xc)
+    T.lib.lde_xfe_batch(T.dev(x), xc, n, T.dev(r), k, 1, 7, m, a, 1)
+    inter = T.host(a).reshape(xc, m, 3)
+    limbs = np.ascontiguousarray(x.reshape(xc, n, 3).transpose(0, 2, 1)).reshape(-1)
+    b = T.empty(3 * m * xc)
+    T.lib.lde_xfe_limbs(T.dev(limbs), xc, n, T.dev(r), k, 1, 7, m, b, 1)
+    lm = T.host(b).reshape(xc, 3, m)
+    assert np.array_equal(inter.transpose(0, 2, 1), lm)
+
+
+# ------------------------------------------------------------------ Merkle
+@pytest.mark.parametrize("logn", [0, 1, 2, 5, 8, 9, 10, 13, 17])
+@pytest.mark.parametrize("layout", [0, 1])
+def test_merkle_tree(T, oracle, logn, layout):
+    n = 1 << logn
+    leaves = oracle.splitmix(50 + logn, 5 * n)
+    nodes = 2 * n - 1 if layout == 0 else 2 * n
+    tree = T.empty(5 * nodes)
+    T.lib.merkle_tree(T.dev(leaves), tree, n, layout)
+    got = T.host(tree).reshape(nodes, 5)
+    if layout == 0:
+        want = oracle.merkle_level_order(leaves)
+        assert np.array_equal(got, want)
+    else:
+        want = oracle.merkle_heap(leaves)
+        assert np.array_equal(got[1:], want[1:])
+    root = T.empty(5)
+    T.lib.merkle_root(T.dev(leaves), root, n)
+    assert np.array_equal(T.host(root), oracle.merkle_heap(leaves)[1])
+
+
+def test_merkle_golden_and_auth_structure(T, oracle, golden):
+    g = golden["merkle"]
+    n = g["n"]
+    leaves = oracle.splitmix(g["seed"], 5 * n)
+    for layout in (0, 1):
+        tree = T.empty(5 * 2 * n)
+        T.lib.merkle_tree(T.dev(leaves), tree, n, layout)
+        for c in golden["auth_structure"]:
+            idx = T.lib.auth_structure_indices(n, c["indices"])
+            out = T.empty(5 * max(1, len(idx)))
+            T.lib.merkle_gather(tree, n, layout, T.dev(idx.astype(np.uint64)), len(idx), out)
+            assert [int(v) for v in T.host(out, 5 * len(idx))] == c["digests"]
+        path_idx = T.lib.auth_path_indices(n, 17)
+        out = T.empty(5 * len(path_idx))
+        T.lib.merkle_gather(tree, n, layout, T.dev(path_idx), len(path_idx), out)
+        assert oracle.verify_path(np.array(g["root"], np.uint64), 17, leaves[85:90],
+                                  T.host(out).reshape(-1, 5))
+
+
+# ------------------------------------------------------------------ commit
+def test_commit_golden(T, oracle, golden):
+    for c in golden["commit"]:
+        n, w, m = c["n"], c["w"], c["m"]
+        t = oracle.table_rowmajor_draw(c["seed"], n, w)
+        cw = T.empty(m * w)
+        tree = T.empty(5 * (2 * m - 1))
+        T.lib.commit(T.dev(t), w, n, None, 0, 1, 7, m, cw, tree, 0)
+        tr = T.host(tree).reshape(2 * m - 1, 5)
+        assert [int(v) for v in tr[-1]] == c["root"], (n, w)
+        assert sha(T.host(cw, m * w)) == c["codewords_sha256"], (n, w)
+        assert sha(tr[:m]) == c["digests_sha256"], (n, w)
+
+
+def test_commit_host_form(T, oracle, golden):
+    c = next(c for c in golden["commit"] if c["n"] == 1024 and c["w"] == 379)
+    t = oracle.table_rowmajor_draw(c["seed"], 1024, 379)
+    root = np.zeros(5, np.uint64)
+    T.lib.commit_host(t, 379, 1024, 4096, root=root)
+    assert [int(v) for v in root] == c["root"]
+
+
+def test_commit_full_size_sampled(T, oracle):
+    """2^20 x 379 -> 2^22: sampled columns/rows/paths against the oracle."""
+    import torch
+
+    n, w, E = 1 << 20, 379, 4
+    m = E * n
+    t = oracle.table_rowmajor_draw(1, n, w)
+    dt = T.dev(t)
+    del t
+    cw = T.empty(m * w)
+    tree = T.empty(5 * (2 * m - 1))
+    T.lib.commit(dt, w, n, None, 0, 1, 7, m, cw, tree, 0)
+    T.sync()
+    x0 = T.host(dt[0:n])
+    x378 = T.host(dt[378 * n:379 * n])
+    assert np.array_equal(T.host(cw[0:m]), oracle.lde_column(x0, m))
+    assert np.array_equal(T.host(cw[378 * m:379 * m]), oracle.lde_column(x378, m))
+    rng = np.random.default_rng(3)
+    rows = np.concatenate([[0, 1, m - 1], rng.integers(0, m, 40)])
+    ridx = torch.tensor(rows, dtype=torch.int64, device="cuda")
+    view = cw.view(w, m)
+    sample = T.host(view[:, ridx].t().contiguous()).reshape(len(rows), w)
+    tr = tree.view(-1, 5)
+    leaves = T.host(tr[ridx].contiguous()).reshape(len(rows), 5)
+    for i, r in enumerate(rows):
+        assert np.array_equal(leaves[i], oracle.hash_varlen(sample[i])), r
+    root = T.host(tr[2 * m - 2])
+    for r in rows[:8]:
+        pidx = T.lib.auth_path_indices(m, int(r))
+        out = T.empty(5 * len(pidx))
+        T.lib.merkle_gather(tree, m, 0, T.dev(pidx), len(pidx), out)
+        assert oracle.verify_path(root, int(r), leaves[list(rows).index(r)],
+                                  T.host(out).reshape(-1, 5))

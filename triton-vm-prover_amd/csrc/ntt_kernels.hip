@@ -1,0 +1,572 @@
+// Batched Goldilocks NTT and coset low-degree extension for gfx950.
+//
+// Math restated from src/ntt/ntt.cpp:26-131 and src/lde/lde_randomized.cpp:
+// the LDE of a column x of length N onto the coset g*<w_M>, M = E*N, is
+//   out[E*q + s] = NTT_N(f_s)[q],  f_s[r] = sum_{i = r mod N} p_i (g w_M^s)^i
+// where p = INTT_N(x) * o^-i + (X^N - o^N) * rand(X) (o = trace offset).  The
+// E cosets replace the reference's zero-pad + size-M NTT (same exact values,
+// a quarter of the butterflies at E = 4).
+//
+// Each size-N transform is a four-step split N = N1 * N2 (N1 = 2^a):
+//   step 1: N1-point DFTs down the columns of the N1 x N2 view, twiddle w_N^(n2 k1)
+//   step 2: N2-point DFTs along rows (k1 fixed)
+// The LDE fuses INTT step 2, the coset scaling of all E cosets and forward
+// step 1 into one kernel (k_lde_step2), so a column is read from HBM three
+// times and written three times, with d_out as the only working memory:
+//   K1  trace            -> Y   at out[E*N2*k1 + n2]
+//   K2  Y (rows k1)      -> Z_s at out[E*(u1 + N2*k1) + s]   (in place per row)
+//   K3  Z (u1-tile, all s) -> codewords at out[E*(u1 + N2*u2) + s] (in place)
+// Sub-DFTs of length L <= 4096 run in LDS as Stockham radix-16 passes: each
+// thread holds R values, applies w_(Ns*R)^(k*r) from a table and an R-point
+// DFT whose twiddles are powers of two (w_64 = 2^39), natural order in/out.
+#include <mutex>
+#include <map>
+#include <tuple>
+
+#include "tvh_internal.hpp"
+
+namespace tvh {
+
+constexpr int TPB = 256;     // threads per block of every NTT kernel
+constexpr int TILE = 8192;   // u64 elements per block (64 KB before padding)
+
+__device__ __forceinline__ int pidx(int t, int i, int Lp) { return t * Lp + i + (i >> 4); }
+__host__ __device__ inline int padded_len(int L) { return L + (L >> 4) + 1; }
+
+template <int LOGR>
+__device__ __forceinline__ constexpr int brev(int i) {
+    int r = 0;
+    for (int b = 0; b < LOGR; b++)
+        if (i & (1 << b)) r |= 1 << (LOGR - 1 - b);
+    return r;
+}
+
+// R-point DFT in registers, natural order in and out.  w_len^j = 2^(39*(64/len)*j).
+template <int LOGR, bool INV>
+__device__ __forceinline__ void dft_reg(u64* v) {
+    constexpr int R = 1 << LOGR;
+    u64 a[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) a[i] = v[brev<LOGR>(i)];
+#pragma unroll
+    for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+        for (int i = 0; i < R; i += len) {
+#pragma unroll
+            for (int j = 0; j < len / 2; j++) {
+                int e = (39 * (64 / len) * j) % 192;
+                if (INV) e = (192 - e) % 192;
+                u64 u = a[i + j];
+                u64 t = gl::mul_pow2_rt(a[i + j + len / 2], (u32)e);
+                a[i + j] = gl::add(u, t);
+                a[i + j + len / 2] = gl::sub(u, t);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) v[i] = a[i];
+}
+
+template <int LOGR, bool INV>
+__device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT, int Ns,
+                                              const u64* __restrict__ r4096) {
+    constexpr int R = 1 << LOGR;
+    constexpr int MAXIT = TILE / (R * TPB);
+    const int per = 1 << (logL - LOGR);
+    const int total = NT << (logL - LOGR);
+    const int step = 4096 / (Ns * R);
+    u64 v[MAXIT][R];
+#pragma unroll
+    for (int it = 0; it < MAXIT; it++) {
+        int w = threadIdx.x + it * TPB;
+        if (w < total) {
+            int t = w >> (logL - LOGR);
+            int j = w & (per - 1);
+            int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < R; r++) v[it][r] = buf[pidx(t, j + r * per, Lp)];
+            if (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[it][r] = gl::mul(v[it][r], r4096[k * r * step]);
+            }
+            dft_reg<LOGR, INV>(v[it]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < MAXIT; it++) {
+        int w = threadIdx.x + it * TPB;
+        if (w < total) {
+            int j = (w) & (per - 1);
+            int t = w >> (logL - LOGR);
+            int k = j & (Ns - 1);
+            int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[pidx(t, base + r * Ns, Lp)] = v[it][r];
+        }
+    }
+    __syncthreads();
+}
+
+// NT transforms of length 2^logL (<= 4096) in LDS; NT * L <= TILE.
+template <bool INV>
+__device__ void lds_dft(u64* buf, int logL, int NT, const u64* __restrict__ r4096) {
+    const int Lp = padded_len(1 << logL);
+    int Ns = 1;
+    int rem = logL;
+    while (rem >= 4) {
+        stockham_pass<4, INV>(buf, logL, Lp, NT, Ns, r4096);
+        Ns <<= 4;
+        rem -= 4;
+    }
+    if (rem == 3) stockham_pass<3, INV>(buf, logL, Lp, NT, Ns, r4096);
+    else if (rem == 2) stockham_pass<2, INV>(buf, logL, Lp, NT, Ns, r4096);
+    else if (rem == 1) stockham_pass<1, INV>(buf, logL, Lp, NT, Ns, r4096);
+}
+
+// w_N^(+-x) for x < N via the two-level 2^24 tables.
+__device__ __forceinline__ u64 twiddle(const Roots& rt, int dir, u32 x, u32 logN) {
+    u32 y = x << (kMaxLogN - logN);
+    return gl::mul(rt.lo[dir][y & 4095], rt.hi[dir][y >> 12]);
+}
+
+// ---------------------------------------------------------------- step 1
+// Transforms s = (c, n2), element n1 at src[c*sstride + N2*n1 + n2]; DFT over n1
+// (inverse if INV) -> k1; times w_N^(+-n2*k1); to dst[c*dstride + E*N2*k1 + n2].
+template <bool INV>
+__global__ void __launch_bounds__(TPB) k_step1(const u64* __restrict__ src, u64 sstride,
+                                               u64* __restrict__ dst, u64 dstride, u32 logN,
+                                               u32 a, u32 E, u64 ntr, u32 logNT, Roots rt) {
+    extern __shared__ u64 lds[];
+    const u32 b = logN - a;
+    const int L = 1 << a;
+    const int Lp = padded_len(L);
+    const int NT = 1 << logNT;
+    const u64 s0 = (u64)blockIdx.x << logNT;
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e & (NT - 1), i = e >> logNT;
+        u64 s = s0 + t;
+        if (s < ntr) {
+            u64 c = s >> b, n2 = s & ((1ull << b) - 1);
+            lds[pidx(t, i, Lp)] = src[c * sstride + ((u64)i << b) + n2];
+        }
+    }
+    __syncthreads();
+    lds_dft<INV>(lds, a, NT, rt.r4096[INV]);
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e & (NT - 1), k1 = e >> logNT;
+        u64 s = s0 + t;
+        if (s < ntr) {
+            u64 c = s >> b, n2 = s & ((1ull << b) - 1);
+            u64 v = lds[pidx(t, k1, Lp)];
+            if (n2 && k1) v = gl::mul(v, twiddle(rt, INV, (u32)(n2 * k1), logN));
+            dst[c * dstride + (((u64)E * k1) << b) + n2] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- plain NTT step 2
+// Rows s = (c, k1): n2 at src[c*N + N2*k1 + n2]; DFT over n2 -> k2 (times N^-1
+// when INV); to dst[c*N + k1 + N1*k2].
+template <bool INV>
+__global__ void __launch_bounds__(TPB) k_ntt_step2(const u64* __restrict__ src,
+                                                   u64* __restrict__ dst, u32 logN, u32 a,
+                                                   u64 nrows, u32 logNT, u64 ninv, Roots rt) {
+    extern __shared__ u64 lds[];
+    const u32 b = logN - a;
+    const int L = 1 << b;
+    const int Lp = padded_len(L);
+    const int NT = 1 << logNT;
+    const u64 s0 = (u64)blockIdx.x << logNT;
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e >> b, i = e & (L - 1);
+        u64 s = s0 + t;
+        if (s < nrows) lds[pidx(t, i, Lp)] = src[(s << b) + i];
+    }
+    __syncthreads();
+    lds_dft<INV>(lds, b, NT, rt.r4096[INV]);
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e & (NT - 1), k2 = e >> logNT;
+        u64 s = s0 + t;
+        if (s < nrows) {
+            u64 c = s >> a, k1 = s & ((1ull << a) - 1);
+            u64 v = lds[pidx(t, k2, Lp)];
+            if (INV) v = gl::mul(v, ninv);
+            dst[(c << logN) + k1 + ((u64)k2 << a)] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- LDE step 2
+struct LdeArgs {
+    u32 logN, a, E, logE;
+    u64 cols, M;
+    const u64* pa;     // pa[s*N1 + k1] = N^-1 * beta_s^k1
+    const u64* pb;     // pb[s*N2 + k2] = beta_s^(N1*k2)
+    u64 delta_n[16];   // (g^N w_E^s - o^N) * N  (or -o^N * N when truncating with E = 1)
+    u64 toff;          // o
+    RandSpec rs;
+};
+
+__device__ __forceinline__ u64 rand_term(const LdeArgs& A, u64 c, u32 s, u64 r) {
+    const RandSpec& rs = A.rs;
+    u32 limb = (u32)(c % rs.group);
+    if (!((rs.limb_mask >> limb) & 1)) return 0;
+    const u64* p = rs.ptr + (c / rs.group) * rs.group_stride + limb * rs.limb_stride +
+                   r * rs.elem_stride;
+    return gl::mul(gl::mul(*p, A.delta_n[s]), gl::pow(A.toff, r));
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_step2(u64* __restrict__ out, LdeArgs A, u32 logNT,
+                                                   Roots rt) {
+    extern __shared__ u64 lds[];
+    const u32 a = A.a, b = A.logN - A.a, E = A.E;
+    const int L = 1 << b;
+    const int Lp = padded_len(L);
+    const int NT = 1 << logNT;           // rows per block
+    const u64 nrows = A.cols << a;
+    const u64 s0 = (u64)blockIdx.x << logNT;
+    // load Y rows into transforms 0..NT-1 (slot s = 0)
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e >> b, i = e & (L - 1);
+        u64 s = s0 + t;
+        if (s < nrows) {
+            u64 c = s >> a, k1 = s & ((1ull << a) - 1);
+            lds[pidx(t, i, Lp)] = out[c * A.M + (((u64)E * k1) << b) + i];
+        }
+    }
+    __syncthreads();
+    lds_dft<true>(lds, b, NT, rt.r4096[1]);
+    // expand: transform (sl*NT + t) <- f_sl for row t
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e >> b, k2 = e & (L - 1);
+        u64 s = s0 + t;
+        if (s < nrows) {
+            u64 c = s >> a, k1 = s & ((1ull << a) - 1);
+            u64 C = lds[pidx(t, k2, Lp)];
+            u64 r = k1 + ((u64)k2 << a);
+            for (u32 sl = 0; sl < E; sl++) {
+                u64 v = C;
+                if (r < A.rs.k) v = gl::add(v, rand_term(A, c, sl, r));
+                u64 sc = gl::mul(A.pa[(sl << a) + k1], A.pb[(sl << b) + k2]);
+                lds[pidx(sl * NT + t, k2, Lp)] = gl::mul(v, sc);
+            }
+        }
+    }
+    __syncthreads();
+    lds_dft<false>(lds, b, NT * E, rt.r4096[0]);
+    // write Z_s[u1][k1] to out[E*(u1 + N2*k1) + s], contiguous over (u1, s)
+    const int RW = L * E;
+    for (int e = threadIdx.x; e < NT * RW; e += TPB) {
+        int t = e / RW, q = e % RW;
+        int u1 = q >> A.logE, sl = q & (E - 1);
+        u64 s = s0 + t;
+        if (s < nrows) {
+            u64 c = s >> a, k1 = s & ((1ull << a) - 1);
+            u64 v = lds[pidx(sl * NT + t, u1, Lp)];
+            if (u1 && k1) v = gl::mul(v, twiddle(rt, 0, (u32)(k1 * u1), A.logN));
+            out[c * A.M + (((u64)E * k1) << b) + q] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- LDE step 3
+// Transforms (c, q = u1*E + s), element k1 at out[c*M + E*N2*k1 + q]; forward
+// DFT over k1 -> u2, written back to out[c*M + E*N2*u2 + q].
+__global__ void __launch_bounds__(TPB) k_lde_step3(u64* __restrict__ out, u32 logN, u32 a,
+                                                   u32 logE, u64 cols, u64 M, u32 logNT,
+                                                   Roots rt) {
+    extern __shared__ u64 lds[];
+    const u32 b = logN - a;
+    const int L = 1 << a;
+    const int Lp = padded_len(L);
+    const int NT = 1 << logNT;
+    const u32 logQ = b + logE;  // transforms per column = E*N2
+    const u64 ntr = cols << logQ;
+    const u64 s0 = (u64)blockIdx.x << logNT;
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e & (NT - 1), i = e >> logNT;
+        u64 s = s0 + t;
+        if (s < ntr) {
+            u64 c = s >> logQ, q = s & ((1ull << logQ) - 1);
+            lds[pidx(t, i, Lp)] = out[c * M + ((u64)i << logQ) + q];
+        }
+    }
+    __syncthreads();
+    lds_dft<false>(lds, a, NT, rt.r4096[0]);
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e & (NT - 1), u2 = e >> logNT;
+        u64 s = s0 + t;
+        if (s < ntr) {
+            u64 c = s >> logQ, q = s & ((1ull << logQ) - 1);
+            out[c * M + ((u64)u2 << logQ) + q] = lds[pidx(t, u2, Lp)];
+        }
+    }
+}
+
+// Constant tables pa/pb of one (logN, a, E, o, g).
+__global__ void k_lde_consts(u64* pa, u64* pb, u32 a, u32 b, u32 E, const u64* beta, u64 ninv) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    u64 n1 = 1ull << a, n2 = 1ull << b;
+    if (i < E * n1) {
+        u64 s = i >> a, k1 = i & (n1 - 1);
+        pa[i] = gl::mul(ninv, gl::pow(beta[s], k1));
+    } else if (i < E * (n1 + n2)) {
+        u64 j = i - E * n1;
+        u64 s = j >> b, k2 = j & (n2 - 1);
+        pb[j] = gl::pow(gl::pow(beta[s], n1), k2);
+    }
+}
+
+// XFE interleaved <-> limb-major.  Interleaved: (c*len + r)*3 + k; limb-major: (3c + k)*len + r.
+__global__ void k_xfe_relayout(const u64* __restrict__ src, u64* __restrict__ dst, u64 cols,
+                               u64 len, int to_limb_major) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cols * len * 3) return;
+    // i indexes the interleaved position
+    u64 k = i % 3, cr = i / 3, r = cr % len, c = cr / len;
+    u64 lm = (3 * c + k) * len + r;
+    if (to_limb_major) dst[lm] = src[i];
+    else dst[i] = src[lm];
+}
+
+// ---------------------------------------------------------------- host side
+
+static u32 choose_split(u32 logN, u32 E) {
+    // a = log N1 <= log N2 (strided step gets the wider tiles); step-2 rows of
+    // all E cosets must fit one tile: E * N2 <= TILE.
+    u32 a = logN / 2;
+    u32 b = logN - a;
+    u32 logE = log2u(E);
+    u32 bmax = 13 - logE;  // log2(TILE) = 13
+    if (b > bmax) {
+        b = bmax;
+        a = logN - b;
+    }
+    return a;
+}
+
+static u32 tile_log_nt(u32 logL, u32 cap_log) {
+    // transforms per block: TILE / max(L, 32), at most 2^cap_log
+    u32 l = logL < 5 ? 5 : logL;
+    u32 t = 13 - l;
+    return t < cap_log ? t : cap_log;
+}
+
+static size_t lds_bytes(u32 logL, u32 logNT) {
+    return ((size_t)padded_len(1 << logL) << logNT) * sizeof(u64);
+}
+
+struct ConstKey {
+    u32 logN, a, E;
+    u64 toff, goff;
+    int dev;
+    bool operator<(const ConstKey& o) const {
+        return std::tie(logN, a, E, toff, goff, dev) <
+               std::tie(o.logN, o.a, o.E, o.toff, o.goff, o.dev);
+    }
+};
+struct ConstVal {
+    u64* pa;
+    u64* pb;
+};
+static std::mutex g_const_mu;
+static std::map<ConstKey, ConstVal> g_consts;
+
+static int lde_consts(DeviceState* ds, u32 logN, u32 a, u32 E, u64 toff, u64 goff, ConstVal* out) {
+    ConstKey key{logN, a, E, toff, goff, ds->device};
+    std::lock_guard<std::mutex> lk(g_const_mu);
+    auto it = g_consts.find(key);
+    if (it != g_consts.end()) {
+        *out = it->second;
+        return TVH_OK;
+    }
+    u32 b = logN - a;
+    u64 n1 = 1ull << a, n2 = 1ull << b;
+    u64 M = (u64)E << logN;
+    u64 wM = gl::root_of_unity(log2u(M));
+    u64 oinv = gl::inv(toff);
+    u64 beta[16];
+    u64 ws = 1;
+    for (u32 s = 0; s < E; s++) {
+        beta[s] = gl::mul(gl::mul(goff, ws), oinv);
+        ws = gl::mul(ws, wM);
+    }
+    u64 ninv = gl::inv(1ull << logN);
+    u64* mem = nullptr;
+    size_t bytes = (E * (n1 + n2) + 16) * sizeof(u64);
+    if (hipMalloc(&mem, bytes) != hipSuccess) return set_error(TVH_ERR_OOM, "lde constants");
+    u64* dbeta = mem + E * (n1 + n2);
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+        return set_error(TVH_ERR_DEVICE, "stream create");
+    int rc = TVH_OK;
+    if (hipMemcpyAsync(dbeta, beta, sizeof(beta), hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = set_error(TVH_ERR_DEVICE, "copy beta");
+    if (!rc) {
+        u64 total = E * (n1 + n2);
+        hipLaunchKernelGGL(k_lde_consts, dim3((u32)((total + 255) / 256)), dim3(256), 0, st, mem,
+                           mem + E * n1, a, b, E, dbeta, ninv);
+        rc = check_launch();
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = set_error(TVH_ERR_DEVICE, "sync");
+    (void)hipStreamDestroy(st);
+    if (rc) {
+        (void)hipFree(mem);
+        return rc;
+    }
+    ConstVal v{mem, mem + E * n1};
+    g_consts[key] = v;
+    *out = v;
+    return TVH_OK;
+}
+
+int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff, u64 m,
+               bool truncate, u64* out, hipStream_t st) {
+    if (cols == 0) return TVH_OK;
+    if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
+    if (m < n) return set_error(TVH_ERR_INVALID_ARG, "target_len < trace_len");
+    u64 E64 = m / n;
+    if (E64 > 16) return set_error(TVH_ERR_UNSUPPORTED, "expansion > 16");
+    if (rs.k > n) return set_error(TVH_ERR_UNSUPPORTED, "rand_len > trace_len");
+    if (toff == 0 || toff >= gl::P || goff >= gl::P)
+        return set_error(TVH_ERR_INVALID_ARG, "offsets must be canonical, trace offset nonzero");
+    u32 logN = log2u(n), E = (u32)E64, logE = log2u(E64);
+    if (logN > 23) return set_error(TVH_ERR_UNSUPPORTED, "trace_len > 2^23");
+    DeviceState* ds = current_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    u32 a = choose_split(logN, E);
+    u32 b = logN - a;
+    if (a > 12) return set_error(TVH_ERR_UNSUPPORTED, "split");
+    ConstVal cv;
+    int rc = lde_consts(ds, logN, a, E, toff, goff, &cv);
+    if (rc) return rc;
+
+    // K1: INTT step 1 (transforms (c, n2))
+    {
+        u32 logNT = tile_log_nt(a, 30);
+        u64 ntr = cols << b;
+        u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
+        hipLaunchKernelGGL(k_step1<true>, dim3((u32)blocks), dim3(TPB), lds_bytes(a, logNT), st,
+                           trace, n, out, m, logN, a, E, ntr, logNT, ds->roots);
+        if ((rc = check_launch())) return rc;
+    }
+    // K2: INTT step 2 + coset scaling + forward step 1
+    {
+        LdeArgs A;
+        A.logN = logN;
+        A.a = a;
+        A.E = E;
+        A.logE = logE;
+        A.cols = cols;
+        A.M = m;
+        A.pa = cv.pa;
+        A.pb = cv.pb;
+        A.toff = toff;
+        A.rs = rs;
+        u64 gN = gl::pow(goff, n), oN = gl::pow(toff, n);
+        u64 wE = gl::root_of_unity(logE), ws = 1;
+        u64 nmod = n % gl::P;
+        for (u32 s = 0; s < 16; s++) {
+            u64 d = (truncate && E == 1) ? gl::neg(oN) : gl::sub(gl::mul(gN, ws), oN);
+            A.delta_n[s] = gl::mul(d, nmod);
+            ws = gl::mul(ws, wE);
+        }
+        u32 logNT = 13 - (b + logE);  // NT * E * N2 = TILE
+        u64 nrows = cols << a;
+        if ((1ull << logNT) > nrows) logNT = log2u(nrows);
+        u64 blocks = (nrows + (1ull << logNT) - 1) >> logNT;
+        hipLaunchKernelGGL(k_lde_step2, dim3((u32)blocks), dim3(TPB), lds_bytes(b, logNT + logE),
+                           st, out, A, logNT, ds->roots);
+        if ((rc = check_launch())) return rc;
+    }
+    // K3: forward step 2 for all cosets
+    {
+        u32 logNT = tile_log_nt(a, 30);
+        u64 ntr = cols << (b + logE);
+        u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
+        hipLaunchKernelGGL(k_lde_step3, dim3((u32)blocks), dim3(TPB), lds_bytes(a, logNT), st,
+                           out, logN, a, logE, cols, m, logNT, ds->roots);
+        if ((rc = check_launch())) return rc;
+    }
+    return TVH_OK;
+}
+
+int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st) {
+    if (cols == 0) return TVH_OK;
+    if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
+    u32 logN = log2u(n);
+    if (logN > kMaxLogN) return set_error(TVH_ERR_UNSUPPORTED, "NTT size > 2^24");
+    DeviceState* ds = current_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    u32 a = logN / 2, b = logN - a;
+    if (b > 12) {
+        b = 12;
+        a = logN - b;
+    }
+    // columns processed in groups so that the scratch stays <= 256 MB
+    u64 group = cols;
+    u64 max_group = (256ull << 20) / (n * sizeof(u64));
+    if (max_group < 1) max_group = 1;
+    if (group > max_group) group = max_group;
+    u64* scratch = (u64*)arena(ds, group * n * sizeof(u64), st);
+    if (!scratch) return TVH_ERR_OOM;
+    u64 ninv = gl::inv(n % gl::P);
+    int rc;
+    for (u64 c0 = 0; c0 < cols; c0 += group) {
+        u64 g = cols - c0 < group ? cols - c0 : group;
+        u64* d = data + c0 * n;
+        {
+            u32 logNT = tile_log_nt(a, 30);
+            u64 ntr = g << b;
+            u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
+            if (inverse)
+                hipLaunchKernelGGL(k_step1<true>, dim3((u32)blocks), dim3(TPB),
+                                   lds_bytes(a, logNT), st, d, n, scratch, n, logN, a, 1u, ntr,
+                                   logNT, ds->roots);
+            else
+                hipLaunchKernelGGL(k_step1<false>, dim3((u32)blocks), dim3(TPB),
+                                   lds_bytes(a, logNT), st, d, n, scratch, n, logN, a, 1u, ntr,
+                                   logNT, ds->roots);
+            if ((rc = check_launch())) return rc;
+        }
+        {
+            u32 logNT = tile_log_nt(b, 30);
+            u64 nrows = g << a;
+            if ((1ull << logNT) > nrows) logNT = log2u(nrows);
+            u64 blocks = (nrows + (1ull << logNT) - 1) >> logNT;
+            if (inverse)
+                hipLaunchKernelGGL(k_ntt_step2<true>, dim3((u32)blocks), dim3(TPB),
+                                   lds_bytes(b, logNT), st, scratch, d, logN, a, nrows, logNT,
+                                   ninv, ds->roots);
+            else
+                hipLaunchKernelGGL(k_ntt_step2<false>, dim3((u32)blocks), dim3(TPB),
+                                   lds_bytes(b, logNT), st, scratch, d, logN, a, nrows, logNT,
+                                   ninv, ds->roots);
+            if ((rc = check_launch())) return rc;
+        }
+    }
+    return TVH_OK;
+}
+
+int ntt_set_attributes() {
+    const int bytes = 80 * 1024;
+    const void* fns[] = {(const void*)k_step1<true>, (const void*)k_step1<false>,
+                         (const void*)k_ntt_step2<true>, (const void*)k_ntt_step2<false>,
+                         (const void*)k_lde_step2, (const void*)k_lde_step3};
+    for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+            return set_error(TVH_ERR_DEVICE, "LDS attribute");
+    return TVH_OK;
+}
+
+int launch_xfe_interleave(const u64* src, u64* dst, u64 cols, u64 len, bool to_limb_major,
+                          hipStream_t st) {
+    u64 total = cols * len * 3;
+    if (!total) return TVH_OK;
+    hipLaunchKernelGGL(k_xfe_relayout, dim3((u32)((total + 255) / 256)), dim3(256), 0, st, src,
+                       dst, cols, len, (int)to_limb_major);
+    return check_launch();
+}
+
+}  // namespace tvh

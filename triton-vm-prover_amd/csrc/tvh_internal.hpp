@@ -1,0 +1,92 @@
+// Internal declarations shared by the HIP translation units of libtvh.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstddef>
+
+#include "../../include/tvh.h"
+#include "gl64.hpp"
+#include "tip5_dev.hpp"
+
+namespace tvh {
+
+using gl::u32;
+using gl::u64;
+
+// Twiddle tables of one device.  w = 7^((p-1)/2^24) (the reference's 2^24-th
+// root); entries for direction d (0 forward, 1 inverse, using w^-1):
+//   r4096[d][e] = w_4096^(+-e)               e < 4096
+//   lo[d][e]    = w^(+-e), hi[d][e] = w^(+-4096 e)   e < 4096
+// so w_N^x = lo[x & 4095] * hi[x >> 12] for x < 2^24 after scaling x by 2^24/N.
+struct Roots {
+    const u64* r4096[2];
+    const u64* lo[2];
+    const u64* hi[2];
+};
+
+constexpr u32 kMaxLogN = 24;
+
+struct DeviceState {
+    int device = -1;
+    Roots roots{};
+    u64* table_mem = nullptr;  // backing store of the twiddle tables
+    // growable arena for entry points that need scratch (documented in tvh.h)
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+};
+
+// State of the current device, initialised on first use (thread-safe).
+// Returns nullptr and sets the error on failure.
+DeviceState* current_device();
+
+// Arena of at least `bytes` on the current device (grows; not for concurrent
+// use of the same device from several streams: calls are serialised by the
+// caller's stream order only).
+void* arena(DeviceState* ds, size_t bytes, hipStream_t st);
+
+int set_error(int status, const char* fmt, ...);
+int check_launch();
+
+inline bool is_pow2(u64 x) { return x && !(x & (x - 1)); }
+inline u32 log2u(u64 x) {
+    u32 l = 0;
+    while ((1ull << l) < x) l++;
+    return l;
+}
+
+// ---- kernels launchers (ntt_kernels.hip) ----
+int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st);
+// LDE of `cols` B-field columns (column-major in, column-major out).  rand may
+// be null (k = 0).  truncate selects the XFE evaluation rule.  lift_stride:
+// randomizer of column c is rand[(c / lift_div) * rand_col_stride + ...] with
+// only limbs (c % lift_div) == lift_limb ... (see ntt_kernels.hip)
+struct RandSpec {
+    const u64* ptr = nullptr;  // null: no randomizer
+    u64 k = 0;                 // coefficients per column
+    // address of coefficient i for B-field column c:
+    //   ptr[(c / group) * group_stride + (c % group) * limb_stride + i * elem_stride]
+    // columns with (c % group) not in the mask get no randomizer
+    u32 group = 1;
+    u64 group_stride = 0;
+    u64 limb_stride = 0;
+    u64 elem_stride = 1;
+    u32 limb_mask = 1;  // bit j set: limb j carries a randomizer
+};
+int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff, u64 m,
+               bool truncate, u64* out, hipStream_t st);
+int launch_xfe_interleave(const u64* src, u64* dst, u64 cols, u64 len, bool to_limb_major,
+                          hipStream_t st);
+
+// ---- Tip5 launchers (tip5_kernels.hip) ----
+int launch_hash_rows(const u64* tab, u64 rows, u64 cols, u64* dig, hipStream_t st);
+int launch_permutation(u64* states, u64 num, hipStream_t st);
+int launch_hash_pairs(const u64* l, const u64* r, u64* out, u64 count, hipStream_t st);
+u64 merkle_level_base(u64 n, u32 level, int layout);
+int launch_merkle_levels(u64* tree, u64 n, int layout, hipStream_t st);
+int launch_gather_heap(const u64* tree, u64 n, int layout, const u64* idx, u64 count, u64* out,
+                       hipStream_t st);
+int ntt_set_attributes();
+
+}  // namespace tvh
