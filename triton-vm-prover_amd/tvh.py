@@ -1,18 +1,22 @@
 """ctypes binding of libtvh (include/tvh.h) for Python callers and tests.
 
-Device buffers are torch int64 tensors on a HIP device (the bits are the
-canonical u64 field elements); torch is used only for device memory and
-streams.  Every call goes through the C ABI and raises TVHError on a non-zero
-status.  There is no CPU fallback: without the built library or a GPU the
-calls fail.
+Device buffers are hip_rt.DeviceBuffer objects (u64 elements allocated by the
+same HIP runtime libtvh links); streams are hip_rt.Stream objects or None (the
+null stream).  Every call goes through the C ABI and raises TVHError on a
+non-zero status.  There is no CPU fallback: without the built library or a GPU
+the calls fail.
 """
 from __future__ import annotations
 
 import ctypes as C
 import os
 import re
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+import hip_rt  # noqa: E402
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libtvh.so")
 HEADER = os.path.join(ROOT, "include", "tvh.h")
@@ -70,7 +74,7 @@ _SIGS = {
 
 
 def _ptr(x):
-    """Address of a torch tensor / numpy array / None."""
+    """Address of a DeviceBuffer / numpy array / None."""
     if x is None:
         return None
     if hasattr(x, "data_ptr"):
@@ -82,6 +86,7 @@ class TVH:
     """The loaded library.  Methods mirror the C entry points one to one."""
 
     def __init__(self, path: str = LIB_PATH):
+        hip_rt.hip()  # the runtime libtvh links, loaded first and globally
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} is not built (python -c 'import __graft_entry__ as g; g.build()')")
         self.path = path
@@ -113,10 +118,7 @@ class TVH:
 
     @staticmethod
     def stream_ptr(stream=None):
-        import torch
-
-        s = stream if stream is not None else torch.cuda.current_stream()
-        return s.cuda_stream
+        return None if stream is None else stream.ptr
 
     # -- NTT --------------------------------------------------------------------
     def ntt_forward_batched(self, d, n, cols, stream=None):
@@ -206,27 +208,19 @@ class TVH:
                    _ptr(codewords), _ptr(digests), _ptr(root))
 
 
-# ---- tensor helpers ----------------------------------------------------------
-def to_device(a, device="cuda"):
-    """numpy uint64 -> torch int64 device tensor with the same bits."""
-    import numpy as np
-    import torch
-
-    a = np.ascontiguousarray(a, dtype=np.uint64)
-    return torch.from_numpy(a.view(np.int64)).to(device)
+# ---- buffer helpers ------------------------------------------------------------
+def to_device(a):
+    """numpy uint64 -> DeviceBuffer."""
+    return hip_rt.to_device(a)
 
 
-def to_host(t):
-    """torch int64 tensor -> numpy uint64 with the same bits."""
-    import numpy as np
-
-    return t.detach().cpu().numpy().view(np.uint64)
+def to_host(buf, count=None):
+    """DeviceBuffer -> numpy uint64."""
+    return buf.download(count)
 
 
-def empty(n, device="cuda"):
-    import torch
-
-    return torch.empty(max(1, int(n)), dtype=torch.int64, device=device)
+def empty(n):
+    return hip_rt.DeviceBuffer(max(1, int(n)))
 
 
 def load(path: str = LIB_PATH) -> TVH:
