@@ -77,6 +77,9 @@ def main() -> int:
     ap.add_argument("--width", type=int, default=379)
     ap.add_argument("--expansion", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
+                    help="run one device-to-device copy of this size first (HBM counter "
+                         "calibration: its FETCH_SIZE/WRITE_SIZE are known bytes)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -99,6 +102,12 @@ def main() -> int:
     cw = hip_rt.DeviceBuffer(m * w)
     tree = hip_rt.DeviceBuffer(5 * (2 * m - 1))
     stream = hip_rt.Stream()
+    if args.calibrate_copy_gib > 0:
+        nb = int(args.calibrate_copy_gib * (1 << 30)) // 8
+        a, b = hip_rt.DeviceBuffer(nb), hip_rt.DeviceBuffer(nb)
+        hip_rt.check(hip_rt.hip().hipMemcpy(b.ptr, a.ptr, 8 * nb, hip_rt.D2D), "calibration copy")
+        hip_rt.synchronize()
+        del a, b
 
     def step(ev=None):
         if ev:
@@ -161,7 +170,7 @@ def main() -> int:
                  "permutations_per_s": perms / ((hash_ms + merkle_ms) / 1000)},
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(12, w, E)
+        out["cpu_baseline"] = cpu_baseline(13, w, E)
     print(json.dumps(out))
     return 0
 

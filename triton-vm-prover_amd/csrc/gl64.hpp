@@ -23,40 +23,66 @@ typedef uint32_t u32;
 constexpr u64 P = 0xFFFFFFFF00000001ull;
 constexpr u64 EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
-__host__ __device__ __forceinline__ u64 canon(u64 x) { return x >= P ? x - P : x; }
+__host__ __device__ __forceinline__ u64 mk64(u32 lo, u32 hi) { return ((u64)hi << 32) | lo; }
+
+// Explicit 32-bit carry chains (v_add_co/v_addc, v_sub_co/v_subb on gfx950):
+// full-rate VALU ops, no 64-bit compares.  The same code serves the host.
+
+// any u64 -> canonical
+__host__ __device__ __forceinline__ u64 canon(u64 x) {
+    u32 k, c;
+    u32 t0 = __builtin_addc((u32)x, 0xFFFFFFFFu, 0u, &k);  // x + EPS carries out iff x >= p
+    u32 t1 = __builtin_addc((u32)(x >> 32), 0u, k, &c);
+    return c ? mk64(t0, t1) : x;
+}
 
 // a, b canonical -> canonical
 __host__ __device__ __forceinline__ u64 add(u64 a, u64 b) {
-    u64 s = a + b;
-    // carry (s < a) or s >= p: subtract p, i.e. add EPS modulo 2^64
-    return (s < a || s >= P) ? s + EPS : s;
+    u32 c0, c1, c2, c3;
+    u32 s0 = __builtin_addc((u32)a, (u32)b, 0u, &c0);
+    u32 s1 = __builtin_addc((u32)(a >> 32), (u32)(b >> 32), c0, &c1);
+    u32 t0 = __builtin_addc(s0, 0xFFFFFFFFu, 0u, &c2);
+    u32 t1 = __builtin_addc(s1, 0u, c2, &c3);
+    return (c1 | c3) ? mk64(t0, t1) : mk64(s0, s1);
 }
 
 // a, b canonical -> canonical
 __host__ __device__ __forceinline__ u64 sub(u64 a, u64 b) {
-    u64 d = a - b;
-    return (a < b) ? d - EPS : d;
+    u32 b0, b1, b2, b3;
+    u32 d0 = __builtin_subc((u32)a, (u32)b, 0u, &b0);
+    u32 d1 = __builtin_subc((u32)(a >> 32), (u32)(b >> 32), b0, &b1);
+    u32 e0 = __builtin_subc(d0, 0xFFFFFFFFu, 0u, &b2);   // borrow: add p = subtract EPS
+    u32 e1 = __builtin_subc(d1, 0u, b2, &b3);
+    return b1 ? mk64(e0, e1) : mk64(d0, d1);
 }
-
-// a canonical -> canonical
-__host__ __device__ __forceinline__ u64 neg(u64 a) { return a ? P - a : 0; }
 
 // a any u64, b canonical -> lazy (any u64).  Used to add round constants.
 __host__ __device__ __forceinline__ u64 add_lazy(u64 a, u64 b) {
-    u64 s = a + b;
-    return (s < a) ? s + EPS : s;  // a + b < 2^65 - 2^32: one fold suffices
+    u32 c0, c1, c2, c3;
+    u32 s0 = __builtin_addc((u32)a, (u32)b, 0u, &c0);
+    u32 s1 = __builtin_addc((u32)(a >> 32), (u32)(b >> 32), c0, &c1);
+    // carry: subtract p = add EPS; a + b < 2^65 - 2^32 so this cannot carry again
+    u32 t0 = __builtin_addc(s0, c1 ? 0xFFFFFFFFu : 0u, 0u, &c2);
+    u32 t1 = __builtin_addc(s1, 0u, c2, &c3);
+    return mk64(t0, t1);
 }
 
-// 128-bit value (hi:lo) -> lazy u64 congruent to it.
+// 128-bit value (hi:lo) -> lazy u64 congruent to it:
+// lo + hl*2^64 + hh*2^96 = lo + hl*(2^32 - 1) - hh  (mod p).
+// (Measured on gfx950: the compiler's compare/select form of this fold beats an
+// explicit carry chain, whose SGPR carries add wait states.)
 __host__ __device__ __forceinline__ u64 fold128(u64 lo, u64 hi) {
     u32 hh = (u32)(hi >> 32), hl = (u32)hi;
-    u64 t = lo - hh;                       // + hh * 2^96 = - hh
+    u64 t = lo - hh;
     if (lo < hh) t -= EPS;                 // borrow: add p
-    u64 u = ((u64)hl << 32) - hl;          // hl * 2^64 = hl * (2^32 - 1)
+    u64 u = ((u64)hl << 32) - hl;          // hl * (2^32 - 1) < 2^64
     u64 r = t + u;
     if (r < u) r += EPS;                   // carry: subtract p
     return r;
 }
+
+// a canonical -> canonical
+__host__ __device__ __forceinline__ u64 neg(u64 a) { return a ? P - a : 0; }
 
 __host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -83,30 +109,57 @@ __host__ __device__ __forceinline__ u64 mul_lazy(u64 a, u64 b) {
 // a, b any u64 -> canonical
 __host__ __device__ __forceinline__ u64 mul(u64 a, u64 b) { return canon(mul_lazy(a, b)); }
 
+// 2^e mod p for 0 <= e < 192 (compile-time usable)
+__host__ __device__ constexpr u64 pow2_mod_p(int e) {
+    u64 r = 1;
+    for (int i = 0; i < e; i++) r = (r >= P - r) ? r - (P - r) : r + r;
+    return r;
+}
+
+// 64x64 -> 128 product from four v_mad_u64_u32 and a carry chain.
+__host__ __device__ __forceinline__ void mul_wide_cc(u64 a, u64 b, u64& lo, u64& hi) {
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    u64 p00 = (u64)a0 * b0, p01 = (u64)a0 * b1, p10 = (u64)a1 * b0, p11 = (u64)a1 * b1;
+    u32 k0, K, k2, k3, k4;
+    u32 m0 = __builtin_addc((u32)p01, (u32)p10, 0u, &k0);
+    u32 m1 = __builtin_addc((u32)(p01 >> 32), (u32)(p10 >> 32), k0, &K);   // mid = p01 + p10 (65 bits)
+    u32 l1 = __builtin_addc((u32)(p00 >> 32), m0, 0u, &k2);
+    u32 h0 = __builtin_addc((u32)p11, m1, k2, &k3);
+    u32 h1 = __builtin_addc((u32)(p11 >> 32), K, k3, &k4);
+    lo = mk64((u32)p00, l1);
+    hi = mk64(h0, h1);
+}
+
+// a, b any u64 -> lazy, carry-chain product
+__host__ __device__ __forceinline__ u64 mul_lazy_cc(u64 a, u64 b) {
+    u64 lo, hi;
+    mul_wide_cc(a, b, lo, hi);
+    return fold128(lo, hi);
+}
+
+// x any u64, 0 < E <= 32 -> lazy x * 2^E: lo = x << E, hl = x >> (64 - E) < 2^32
+template <int E>
+__host__ __device__ __forceinline__ u64 mul_pow2_small(u64 x) {
+    static_assert(E > 0 && E <= 32, "small shift");
+    u64 lo = x << E;
+    u32 hl = (u32)(x >> (64 - E));
+    u32 k0, k1, c0, c1, c2, c3;
+    u32 u0 = __builtin_subc(0u, hl, 0u, &k0);
+    u32 u1 = __builtin_subc(hl, 0u, k0, &k1);
+    u32 r0 = __builtin_addc((u32)lo, u0, 0u, &c0);
+    u32 r1 = __builtin_addc((u32)(lo >> 32), u1, c0, &c1);
+    r0 = __builtin_addc(r0, c1 ? 0xFFFFFFFFu : 0u, 0u, &c2);
+    r1 = __builtin_addc(r1, 0u, c2, &c3);
+    return mk64(r0, r1);
+}
+
 // x any u64 -> lazy x * 2^32
 __host__ __device__ __forceinline__ u64 mul_2_32(u64 x) {
     // x = x1*2^32 + x0 ; x*2^32 = x0*2^32 + x1*2^64 = x0*2^32 + x1*(2^32-1)
     return fold128(x << 32, x >> 32);
 }
 
-// x canonical, K compile-time in [0, 192) -> canonical x * 2^K
-template <int K>
-__host__ __device__ __forceinline__ u64 mul_pow2(u64 x) {
-    static_assert(K >= 0 && K < 192, "shift out of range");
-    if constexpr (K == 0) {
-        return x;
-    } else if constexpr (K >= 96) {
-        return neg(mul_pow2<K - 96>(x));   // 2^96 = -1
-    } else if constexpr (K < 64) {
-        return canon(fold128(x << K, x >> (64 - K)));
-    } else {
-        // 2^K = 2^(K-64) * 2^64 ; 2^64 = 2^32 - 1
-        u64 y = (K == 64) ? x : canon(fold128(x << (K - 64), x >> (128 - K)));
-        return sub(canon(mul_2_32(y)), y);
-    }
-}
-
-// x canonical, k runtime in [0, 192) -> canonical x * 2^k
+// x canonical, k runtime in [0, 192) -> canonical x * 2^k (host and tables)
 __host__ __device__ __forceinline__ u64 mul_pow2_rt(u64 x, u32 k) {
     bool negate = k >= 96;
     if (negate) k -= 96;

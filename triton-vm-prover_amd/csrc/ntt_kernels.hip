@@ -27,8 +27,9 @@
 
 namespace tvh {
 
-constexpr int TPB = 256;     // threads per block of every NTT kernel
-constexpr int TILE = 8192;   // u64 elements per block (64 KB before padding)
+constexpr int TPB = 1024;    // threads per block of every NTT kernel (16 waves)
+constexpr int TILE = 16384;  // u64 elements per block: 16 per thread, 128 KB before padding
+constexpr int LOG_TILE = 14;
 
 __device__ __forceinline__ int pidx(int t, int i, int Lp) { return t * Lp + i + (i >> 4); }
 __host__ __device__ inline int padded_len(int L) { return L + (L >> 4) + 1; }
@@ -41,7 +42,21 @@ __device__ __forceinline__ constexpr int brev(int i) {
     return r;
 }
 
-// R-point DFT in registers, natural order in and out.  w_len^j = 2^(39*(64/len)*j).
+// x canonical times the twiddle 2^e, e in [0, 96) known after unrolling ->
+// canonical.  e <= 32 is a shift and one fold; larger e multiply by the
+// constant 2^e mod p.
+__device__ __forceinline__ u64 mul_pow2_c(u64 x, int e) {
+    switch (e) {
+        case 0: return x;
+        case 12: return gl::canon(gl::mul_pow2_small<12>(x));
+        case 24: return gl::canon(gl::mul_pow2_small<24>(x));
+        default: return gl::mul(x, gl::pow2_mod_p(e));
+    }
+}
+
+// R-point DFT in registers, natural order in and out: radix-2 DIT on the
+// bit-reversed input.  w_len^j = 2^(39*(64/len)*j mod 192); a twiddle 2^e with
+// e >= 96 is -2^(e-96), folded into the butterfly by swapping add and sub.
 template <int LOGR, bool INV>
 __device__ __forceinline__ void dft_reg(u64* v) {
     constexpr int R = 1 << LOGR;
@@ -56,10 +71,16 @@ __device__ __forceinline__ void dft_reg(u64* v) {
             for (int j = 0; j < len / 2; j++) {
                 int e = (39 * (64 / len) * j) % 192;
                 if (INV) e = (192 - e) % 192;
+                bool negw = e >= 96;
                 u64 u = a[i + j];
-                u64 t = gl::mul_pow2_rt(a[i + j + len / 2], (u32)e);
-                a[i + j] = gl::add(u, t);
-                a[i + j + len / 2] = gl::sub(u, t);
+                u64 t = mul_pow2_c(a[i + j + len / 2], negw ? e - 96 : e);
+                if (negw) {
+                    a[i + j] = gl::sub(u, t);
+                    a[i + j + len / 2] = gl::add(u, t);
+                } else {
+                    a[i + j] = gl::add(u, t);
+                    a[i + j + len / 2] = gl::sub(u, t);
+                }
             }
         }
     }
@@ -75,6 +96,7 @@ __device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT
     const int per = 1 << (logL - LOGR);
     const int total = NT << (logL - LOGR);
     const int step = 4096 / (Ns * R);
+    static_assert(MAXIT >= 1, "radix larger than the per-thread element count");
     u64 v[MAXIT][R];
 #pragma unroll
     for (int it = 0; it < MAXIT; it++) {
@@ -110,7 +132,7 @@ __device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT
 
 // NT transforms of length 2^logL (<= 4096) in LDS; NT * L <= TILE.
 template <bool INV>
-__device__ void lds_dft(u64* buf, int logL, int NT, const u64* __restrict__ r4096) {
+__device__ __forceinline__ void lds_dft(u64* buf, int logL, int NT, const u64* __restrict__ r4096) {
     const int Lp = padded_len(1 << logL);
     int Ns = 1;
     int rem = logL;
@@ -201,6 +223,8 @@ __global__ void __launch_bounds__(TPB) k_ntt_step2(const u64* __restrict__ src,
 struct LdeArgs {
     u32 logN, a, E, logE;
     u64 cols, M;
+    const u64* src;        // rows of K2: src[c*src_col_stride + k1*src_row_stride + n2]
+    u64 src_col_stride, src_row_stride;
     const u64* pa;     // pa[s*N1 + k1] = N^-1 * beta_s^k1
     const u64* pb;     // pb[s*N2 + k2] = beta_s^(N1*k2)
     u64 delta_n[16];   // (g^N w_E^s - o^N) * N  (or -o^N * N when truncating with E = 1)
@@ -217,7 +241,7 @@ __device__ __forceinline__ u64 rand_term(const LdeArgs& A, u64 c, u32 s, u64 r) 
     return gl::mul(gl::mul(*p, A.delta_n[s]), gl::pow(A.toff, r));
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_step2(u64* __restrict__ out, LdeArgs A, u32 logNT,
+__global__ void __launch_bounds__(TPB) k_lde_step2(u64* out, LdeArgs A, u32 logNT,
                                                    Roots rt) {
     extern __shared__ u64 lds[];
     const u32 a = A.a, b = A.logN - A.a, E = A.E;
@@ -232,7 +256,7 @@ __global__ void __launch_bounds__(TPB) k_lde_step2(u64* __restrict__ out, LdeArg
         u64 s = s0 + t;
         if (s < nrows) {
             u64 c = s >> a, k1 = s & ((1ull << a) - 1);
-            lds[pidx(t, i, Lp)] = out[c * A.M + (((u64)E * k1) << b) + i];
+            lds[pidx(t, i, Lp)] = A.src[c * A.src_col_stride + k1 * A.src_row_stride + i];
         }
     }
     __syncthreads();
@@ -256,9 +280,10 @@ __global__ void __launch_bounds__(TPB) k_lde_step2(u64* __restrict__ out, LdeArg
     __syncthreads();
     lds_dft<false>(lds, b, NT * E, rt.r4096[0]);
     // write Z_s[u1][k1] to out[E*(u1 + N2*k1) + s], contiguous over (u1, s)
-    const int RW = L * E;
+    const int logRW = b + A.logE;  // row of all cosets: E * N2 elements
+    const int RW = 1 << logRW;
     for (int e = threadIdx.x; e < NT * RW; e += TPB) {
-        int t = e / RW, q = e % RW;
+        int t = e >> logRW, q = e & (RW - 1);
         int u1 = q >> A.logE, sl = q & (E - 1);
         u64 s = s0 + t;
         if (s < nrows) {
@@ -333,23 +358,20 @@ __global__ void k_xfe_relayout(const u64* __restrict__ src, u64* __restrict__ ds
 // ---------------------------------------------------------------- host side
 
 static u32 choose_split(u32 logN, u32 E) {
-    // a = log N1 <= log N2 (strided step gets the wider tiles); step-2 rows of
-    // all E cosets must fit one tile: E * N2 <= TILE.
-    u32 a = logN / 2;
-    u32 b = logN - a;
+    // N2 (the contiguous row step) as long as one tile allows for all E cosets
+    // (E * N2 <= TILE, N2 <= 4096); N1 = N / N2 takes the strided step, whose
+    // HBM segments are TILE / N1 elements wide.
     u32 logE = log2u(E);
-    u32 bmax = 13 - logE;  // log2(TILE) = 13
-    if (b > bmax) {
-        b = bmax;
-        a = logN - b;
-    }
-    return a;
+    u32 bmax = LOG_TILE - logE;
+    if (bmax > 12) bmax = 12;
+    u32 b = logN < bmax ? logN : bmax;
+    return logN - b;
 }
 
 static u32 tile_log_nt(u32 logL, u32 cap_log) {
     // transforms per block: TILE / max(L, 32), at most 2^cap_log
     u32 l = logL < 5 ? 5 : logL;
-    u32 t = 13 - l;
+    u32 t = LOG_TILE - l;
     return t < cap_log ? t : cap_log;
 }
 
@@ -442,8 +464,10 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
     int rc = lde_consts(ds, logN, a, E, toff, goff, &cv);
     if (rc) return rc;
 
-    // K1: INTT step 1 (transforms (c, n2))
-    {
+    // K1: INTT step 1 (transforms (c, n2)); a size-1 step is the identity
+    // map trace -> Y, so K2 then reads the trace itself.
+    const bool k1_needed = a > 0;
+    if (k1_needed) {
         u32 logNT = tile_log_nt(a, 30);
         u64 ntr = cols << b;
         u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
@@ -472,16 +496,20 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
             A.delta_n[s] = gl::mul(d, nmod);
             ws = gl::mul(ws, wE);
         }
-        u32 logNT = 13 - (b + logE);  // NT * E * N2 = TILE
+        u32 logNT = LOG_TILE - (b + logE);  // NT * E * N2 = TILE
+        if (b < 5) logNT = LOG_TILE - 5 - logE;
         u64 nrows = cols << a;
         if ((1ull << logNT) > nrows) logNT = log2u(nrows);
         u64 blocks = (nrows + (1ull << logNT) - 1) >> logNT;
+        A.src = k1_needed ? out : trace;
+        A.src_col_stride = k1_needed ? m : n;
+        A.src_row_stride = k1_needed ? ((u64)E << b) : 0;
         hipLaunchKernelGGL(k_lde_step2, dim3((u32)blocks), dim3(TPB), lds_bytes(b, logNT + logE),
                            st, out, A, logNT, ds->roots);
         if ((rc = check_launch())) return rc;
     }
-    // K3: forward step 2 for all cosets
-    {
+    // K3: forward step 2 for all cosets (identity when N1 = 1)
+    if (a > 0) {
         u32 logNT = tile_log_nt(a, 30);
         u64 ntr = cols << (b + logE);
         u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
@@ -550,7 +578,7 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st)
 }
 
 int ntt_set_attributes() {
-    const int bytes = 80 * 1024;
+    const int bytes = 150 * 1024;
     const void* fns[] = {(const void*)k_step1<true>, (const void*)k_step1<false>,
                          (const void*)k_ntt_step2<true>, (const void*)k_ntt_step2<false>,
                          (const void*)k_lde_step2, (const void*)k_lde_step3};
