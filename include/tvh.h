@@ -199,6 +199,14 @@ int tvh_lde_batch_host(const uint64_t* trace, size_t num_cols, size_t trace_len,
                        const uint64_t* rand, size_t rand_len, uint64_t trace_offset,
                        uint64_t target_offset, size_t target_len, uint64_t* out);
 
+/* XFE form of tvh_lde_batch_host in the interleaved layout of
+ * tvh_lde_xfe_batch_dev (RandomizedLDE::extend_xfield_column_with_randomizer /
+ * _with_xfield_randomizer, src/lde/lde_randomized.cpp:179-395). */
+int tvh_lde_xfe_batch_host(const uint64_t* xfe_trace, size_t num_cols, size_t trace_len,
+                           const uint64_t* rand, size_t rand_len, uint64_t trace_offset,
+                           uint64_t target_offset, size_t target_len, uint64_t* xfe_out,
+                           int use_xfe_randomizers);
+
 /* Replace Backend::tip5_permutation_batch / hash_pairs on host buffers. */
 int tvh_tip5_permutation_batch_host(uint64_t* states, size_t num_states);
 int tvh_hash_pairs_host(const uint64_t* left, const uint64_t* right, uint64_t* out,

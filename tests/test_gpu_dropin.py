@@ -1,0 +1,77 @@
+"""The drop-in C++ classes (include/triton_vm_amd/*.hpp, the reference's NTT / LDE /
+RandomizedLDE / LDETable / Tip5 / MerkleTree interfaces) on the GPU, checked against
+the golden fixtures written from the reference's own CPU code and the oracle.
+
+tests/native/dropin_test.cpp runs the classes and prints JSON lines; this test
+parses them.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(ROOT, "triton-vm-prover_amd", "_build", "dropin_test")
+
+
+@pytest.fixture(scope="module")
+def out():
+    if not os.path.exists(BIN):
+        pytest.fail("dropin_test is not built (run __graft_entry__.build())")
+    r = subprocess.run([BIN], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    d = {}
+    for line in r.stdout.splitlines():
+        d.update(json.loads(line))
+    return d
+
+
+def test_ntt(out, oracle):
+    assert out["ntt_kat"] == [5, 1125899906842625, 18446744069414584318, 18445618169507741698]
+    x = oracle.splitmix(4006, 64)
+    assert out["ntt64_fwd"] == [int(v) for v in oracle.ntt(x)]
+    assert out["ntt64_roundtrip"] is True
+    assert out["ntt_non_pow2_throws"] is True
+    assert out["lde_matches_coset_eval"] is True
+
+
+def test_randomized_lde(out, golden):
+    c = next(c for c in golden["lde_column"] if c["n"] == 16 and c["m"] == 64 and c["k"] == 3)
+    assert out["rlde_16_64_k3"] == c["out"]
+    c = next(c for c in golden["lde_xfe_column"]
+             if c["n"] == 16 and c["m"] == 64 and c["k"] == 3 and c["xfe_rand"] == 1)
+    assert out["xlde_16_64_k3_x"] == c["out"]
+    assert out["zerofier_vanishes"] is True
+
+
+def test_tip5(out):
+    assert out["hash10_kat"] == [10869784347448351760, 1853783032222938415, 6856460589287344822,
+                                 17178399545409290325, 7650660984651717733]
+    assert out["hash_varlen_kat_sum"] == [7610004073009036015, 5725198067541094245,
+                                          4721320565792709122, 1732504843634706218,
+                                          259800783350288362]
+    assert out["hash_row_empty_is_zero"] is True
+
+
+def test_merkle(out, golden):
+    assert out["merkle64_root"] == golden["merkle"]["root"]
+    c = next(c for c in golden["auth_structure"] if c["indices"] == [5, 9, 40])
+    assert out["auth_5_9_40"] == c["digests"]
+    assert out["verify_ok"] is True and out["verify_wrong_index"] is False
+    assert out["verify_structure_ok"] is True
+    assert out["merkle_non_pow2_throws"] is True
+    assert out["merkle_leaf_range_throws"] is True
+
+
+def test_commit(out, golden):
+    for n, w in [(8, 7), (16, 9), (1024, 379)]:
+        c = next(c for c in golden["commit"] if c["n"] == n and c["w"] == w)
+        assert out[f"commit_{n}_{w}"] == c["root"], (n, w)
+    c = next(c for c in golden["commit"] if c["n"] == 8 and c["w"] == 7)
+    assert out["two_step_commit_matches"] is True
+    assert out["cw00"] == c["cw_00"]

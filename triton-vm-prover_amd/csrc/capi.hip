@@ -432,6 +432,29 @@ int tvh_lde_batch_host(const uint64_t* trace, size_t cols, size_t n, const uint6
     return hc.sync();
 }
 
+int tvh_lde_xfe_batch_host(const uint64_t* xt, size_t cols, size_t n, const uint64_t* rand,
+                           size_t k, uint64_t toff, uint64_t goff, size_t m, uint64_t* xout,
+                           int xfe_r) {
+    if (!cols) return TVH_OK;
+    if (!xt || !xout || (k && !rand)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    size_t rlen = cols * k * (xfe_r ? 3 : 1);
+    int rc = check_canonical(xt, 3 * cols * n);
+    if (!rc && k) rc = check_canonical(rand, rlen);
+    if (rc) return rc;
+    HostCall hc;
+    if (hc.rc) return hc.rc;
+    u64* dt = hc.alloc(3 * cols * n);
+    u64* dr = hc.alloc(rlen);
+    u64* dout = hc.alloc(3 * cols * m);
+    if (hc.rc) return hc.rc;
+    if ((rc = hc.up(dt, xt, 3 * cols * n)) || (rc = hc.up(dr, rand, rlen))) return rc;
+    if ((rc = tvh_lde_xfe_batch_dev(dt, cols, n, k ? dr : nullptr, k, toff, goff, m, dout, xfe_r,
+                                    hc.st)))
+        return rc;
+    if ((rc = hc.down(xout, dout, 3 * cols * m))) return rc;
+    return hc.sync();
+}
+
 int tvh_tip5_permutation_batch_host(uint64_t* states, size_t num) {
     if (!num) return TVH_OK;
     if (!states) return set_error(TVH_ERR_INVALID_ARG, "null states");

@@ -11,9 +11,10 @@
 //    canonical residues with shifts is bit-identical.  The 256-byte table sits
 //    in LDS.
 //  * MDS: the 16x16 circulant has 16-bit entries, so each state word is split
-//    into 32-bit halves and both halves are multiplied in f64 FMAs: every
-//    partial sum is an integer below 524757 * 2^32 < 2^52 and hence exact.  The
-//    two 52-bit sums are recombined modulo p with integer ops.
+//    into 32-bit halves and both halves are multiplied in f64 (exact: every
+//    partial sum is below 2^52), using the x^8 - 1 / x^8 + 1 split of the
+//    cyclic convolution.  The two 52-bit results are recombined modulo p with
+//    integer ops.
 //  * state words stay "lazy" (any u64 congruent mod p) inside the permutation;
 //    callers canonicalise what they store.
 #pragma once
@@ -54,9 +55,9 @@ __constant__ u64 kRoundConstants[ROUNDS * STATE] = {
     0x8af9d44a5f4fdc6bull, 0xd80219cd97c0d762ull, 0x10c9ceba14148ebbull, 0x539bd4c3f2f24474ull,
 };
 
-// First column of the circulant MDS matrix: SHA-256("Tip5") as 16 LE u16.
-constexpr double kMds[STATE] = {61402, 1108,  28750, 33823, 7454,  43244, 53865, 12034,
-                                56951, 27521, 41351, 40901, 12021, 59689, 26798, 17845};
+// First column of the circulant MDS matrix: SHA-256("Tip5") as 16 LE u16:
+// 61402 1108 28750 33823 7454 43244 53865 12034 56951 27521 41351 40901 12021
+// 59689 26798 17845 (folded into kMdsP / kMdsQ below).
 
 // Fill a 256-byte LDS table with the S-box ((x+1)^3 mod 257) - 1.
 __device__ __forceinline__ void load_lut(uint8_t* lut) {
@@ -91,22 +92,51 @@ __device__ __forceinline__ u64 f2u52(double a) {
     return (u64)__double_as_longlong(b) & 0x000FFFFFFFFFFFFFull;
 }
 
+// Circulant MDS y_r = sum_c M[(r - c) & 15] s_c on the two 32-bit halves of the
+// state, each via x^16 - 1 = (x^8 - 1)(x^8 + 1): a cyclic 8-term product of
+// (s_lo + s_hi) with kMdsP = (m_lo + m_hi)/2 and a negacyclic one of
+// (s_lo - s_hi) with kMdsQ = (m_lo - m_hi)/2 give A and B with y_lo = A + B,
+// y_hi = A - B.  Every partial sum is a multiple of 1/2 below 2^52 in
+// magnitude, so all f64 arithmetic is exact (128 FMAs per half instead of 256).
+constexpr double kMdsP[8] = {(61402 + 56951) / 2.0, (1108 + 27521) / 2.0, (28750 + 41351) / 2.0,
+                             (33823 + 40901) / 2.0, (7454 + 12021) / 2.0, (43244 + 59689) / 2.0,
+                             (53865 + 26798) / 2.0, (12034 + 17845) / 2.0};
+constexpr double kMdsQ[8] = {(61402 - 56951) / 2.0, (1108 - 27521) / 2.0, (28750 - 41351) / 2.0,
+                             (33823 - 40901) / 2.0, (7454 - 12021) / 2.0, (43244 - 59689) / 2.0,
+                             (53865 - 26798) / 2.0, (12034 - 17845) / 2.0};
+
+__device__ __forceinline__ void mds_half(const double x[STATE], double y[STATE]) {
+    double sp[8], sm[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        sp[c] = x[c] + x[c + 8];
+        sm[c] = x[c] - x[c + 8];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        double A = 0.0, B = 0.0;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            A = __fma_rn(kMdsP[(r - c) & 7], sp[c], A);
+            B = __fma_rn(r >= c ? kMdsQ[r - c] : -kMdsQ[r - c + 8], sm[c], B);
+        }
+        y[r] = A + B;
+        y[r + 8] = A - B;
+    }
+}
+
 __device__ __forceinline__ void mds(u64 s[STATE]) {
-    double lo[STATE], hi[STATE];
+    double lo[STATE], hi[STATE], ylo[STATE], yhi[STATE];
 #pragma unroll
     for (int c = 0; c < STATE; c++) {
         lo[c] = (double)(u32)s[c];
         hi[c] = (double)(u32)(s[c] >> 32);
     }
+    mds_half(lo, ylo);
+    mds_half(hi, yhi);
 #pragma unroll
     for (int r = 0; r < STATE; r++) {
-        double al = 0.0, ah = 0.0;
-#pragma unroll
-        for (int c = 0; c < STATE; c++) {
-            al = __fma_rn(kMds[(r - c) & 15], lo[c], al);
-            ah = __fma_rn(kMds[(r - c) & 15], hi[c], ah);
-        }
-        u64 L = f2u52(al), H = f2u52(ah);
+        u64 L = f2u52(ylo[r]), H = f2u52(yhi[r]);
         // value = L + H * 2^32, H = Hhi * 2^32 + Hlo (Hhi < 2^20)
         u64 Hhi = H >> 32;
         u64 t = L + (H << 32);

@@ -65,6 +65,7 @@ _SIGS = {
     "tvh_ntt_forward_host": [_vp, _sz],
     "tvh_ntt_inverse_host": [_vp, _sz],
     "tvh_lde_batch_host": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp],
+    "tvh_lde_xfe_batch_host": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp, _i32],
     "tvh_tip5_permutation_batch_host": [_vp, _sz],
     "tvh_hash_pairs_host": [_vp, _vp, _vp, _sz],
     "tvh_hash_bfield_rows_host": [_vp, _sz, _sz, _vp],
