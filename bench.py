@@ -46,6 +46,18 @@ def random_field(numel: int, seed: int):
     return rng.integers(0, 0xFFFFFFFF00000001, size=numel, dtype=np.uint64)
 
 
+def measured_traffic(log_n: int, width: int, expansion: int):
+    """HBM bytes per LDE launch from the committed rocprof counter summary
+    (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), for the workload it covers."""
+    path = os.path.join(ROOT, "profiles", f"commit_2p{log_n}x{width}_hbm_counters.md")
+    if expansion != 4 or not os.path.exists(path):
+        return None
+    import re
+
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(log_n: int = 12, width: int = 379, expansion: int = 4) -> dict:
     """Reference CPU commit (oracle/_ref) on a 2^log_n x width sample, 1 thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -163,7 +175,8 @@ def main() -> int:
                    "trace_rows": n, "columns": w, "codeword_rows": m},
         "phases_ms": {"lde": lde_ms, "hash_rows": hash_ms, "merkle": merkle_ms},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(args.log_n, w, E),
                      "kernel": "LDE passes (k_step1 + k_lde_step2 + k_lde_step3)",
                      "algorithmic_bytes": lde_bytes},
         "tip5": {"permutations": perms,
