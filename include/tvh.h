@@ -187,6 +187,134 @@ int tvh_commit_dev(const uint64_t* d_trace, size_t num_cols, size_t trace_len,
                    uint64_t target_offset, size_t target_len, uint64_t* d_codewords,
                    uint64_t* d_tree, int layout, void* stream);
 
+/* ---- batched commits of many tables ------------------------------------------ */
+
+/* One table commit of a batch: the arguments of tvh_commit_dev. */
+typedef struct tvh_table_desc {
+    const uint64_t* trace;   /* column-major trace_len x num_cols */
+    size_t num_cols;
+    size_t trace_len;
+    const uint64_t* rand;    /* rand_len coefficients per column, or NULL */
+    size_t rand_len;
+    uint64_t trace_offset;
+    uint64_t target_offset;
+    size_t target_len;
+    uint64_t* codewords;     /* column-major target_len x num_cols */
+    uint64_t* tree;          /* tree in `layout` */
+    int layout;
+} tvh_table_desc;
+
+typedef struct tvh_batch_plan tvh_batch_plan;
+
+/* Builds a plan that commits every table of the list (buffers as given) with
+ * one launch per phase over all tables.  Uploads the per-phase argument
+ * arrays (the only allocation).  Same results as tvh_commit_dev per table.
+ * No reference counterpart: the reference commits tables one at a time
+ * (gpu_stark.cu:2132-2610). */
+int tvh_batch_plan_create(const tvh_table_desc* tables, size_t count, tvh_batch_plan** plan);
+/* Enqueues the plan on `stream` (no allocation, no memset: capturable). */
+int tvh_batch_plan_launch(tvh_batch_plan* plan, void* stream);
+/* First call captures the plan's launches on `stream` (non-NULL) into a
+ * hipGraph; every call launches that graph. */
+int tvh_batch_plan_launch_graph(tvh_batch_plan* plan, void* stream);
+int tvh_batch_plan_destroy(tvh_batch_plan* plan);
+/* create + launch + wait for the stream + destroy. */
+int tvh_commit_batch_dev(const tvh_table_desc* tables, size_t count, void* stream);
+
+/* Synthetic uniform field elements for benchmarks and tests (no reference
+ * counterpart): d[i] = SplitMix64(seed + (i+1) * golden gamma) reduced below p
+ * (values >= p are replaced by v - p). */
+int tvh_fill_uniform_dev(uint64_t* d, size_t count, uint64_t seed, void* stream);
+
+/* Streaming ("frugal") commit: the same result as tvh_commit_dev without
+ * materialising the codewords.  Columns are extended chunk_cols at a time into
+ * a chunk buffer and absorbed into a per-row Tip5 sponge state that stays in
+ * HBM between chunks (the reference's frugal coset loop bounds memory the same
+ * way, gpu_stark.cu:2247-2420).  d_work holds at least
+ * tvh_commit_frugal_workspace_elems(...) u64; the tree is written in the chosen
+ * layout.  No other memory is used. */
+size_t tvh_commit_frugal_workspace_elems(size_t num_cols, size_t trace_len, size_t target_len,
+                                         size_t chunk_cols);
+int tvh_commit_frugal_dev(const uint64_t* d_trace, size_t num_cols, size_t trace_len,
+                          const uint64_t* d_rand, size_t rand_len, uint64_t trace_offset,
+                          uint64_t target_offset, size_t target_len, size_t chunk_cols,
+                          uint64_t* d_work, uint64_t* d_tree, int layout, void* stream);
+
+/* ---- device-resident commit server --------------------------------------- */
+
+/* No reference counterpart as one API: the reference serves proofs by forking
+ * one prover process per request with one job per GPU
+ * (src/prover_server/prover.cpp:459-505; rust/prover_server/src/server.rs:22,
+ * 70-95) and keeps nothing on the device between requests.  Here one process
+ * owns the GPU: jobs queue FIFO and a scheduler thread admits the head job as
+ * soon as a stream slot is free and its buffers fit in the HBM budget
+ * (sub-allocated from one allocation made at create time); admitted jobs run
+ * concurrently on their slot streams; finished jobs keep codewords and tree
+ * on the device until released, for opening queries. */
+typedef struct tvh_server tvh_server;
+
+typedef struct tvh_server_config {
+    size_t hbm_budget_bytes; /* arena for job buffers; 0 = free memory - 16 GiB */
+    int max_inflight;        /* stream slots (jobs on the GPU at once); 0 = 4 */
+    size_t max_queue;        /* queued (not yet admitted) jobs; 0 = unbounded */
+} tvh_server_config;
+
+/* One LDE + commit job (tvh_commit_dev arguments, level-order tree).  Inputs
+ * come from host memory, which must stay valid until the job has finished, or
+ * are drawn on the device from `seed` with tvh_fill_uniform_dev when the
+ * pointer is NULL (trace from seed, randomizers from seed ^ 0x5bd1e995). */
+typedef struct tvh_job_desc {
+    const uint64_t* host_trace; /* column-major trace_len x num_cols, or NULL */
+    const uint64_t* host_rand;  /* rand_len per column (column-major), or NULL */
+    uint64_t seed;
+    size_t num_cols;
+    size_t trace_len;
+    size_t rand_len;
+    uint64_t trace_offset;
+    uint64_t target_offset;
+    size_t target_len;
+} tvh_job_desc;
+
+/* steady-clock seconds of a job's submission, admission and completion */
+typedef struct tvh_job_times {
+    double submit_s;
+    double admit_s;
+    double done_s;
+} tvh_job_times;
+
+enum { TVH_JOB_QUEUED = 0, TVH_JOB_RUNNING = 1, TVH_JOB_DONE = 2, TVH_JOB_FAILED = 3 };
+
+int tvh_server_create(const tvh_server_config* config, tvh_server** server);
+/* Queues a job; never blocks on the GPU.  TVH_ERR_RANGE when the queue is
+ * full.  A job larger than the whole budget fails with TVH_ERR_OOM. */
+int tvh_server_submit(tvh_server* server, const tvh_job_desc* job, uint64_t* job_id);
+int tvh_server_poll(tvh_server* server, uint64_t job_id, int* state);
+/* Blocks until the job has finished; root (5 u64) and times may be NULL. */
+int tvh_server_wait(tvh_server* server, uint64_t job_id, uint64_t* root, tvh_job_times* times);
+/* Opening of a finished job at `count` leaf indices (row indices of the
+ * codewords): rows_out (count x num_cols, row-major, may be NULL) receives the
+ * codeword rows, auth_out the authentication structure digests
+ * (MerkleTree::authentication_structure, merkle_tree.cpp:228-265; heap indices
+ * from tvh_auth_structure_indices), *auth_count their number (5 u64 each).
+ * auth_out may be NULL to query the count. */
+int tvh_server_open(tvh_server* server, uint64_t job_id, const size_t* leaf_indices,
+                    size_t count, uint64_t* rows_out, uint64_t* auth_out, size_t auth_cap,
+                    size_t* auth_count);
+/* Authentication paths (bottom-up siblings, merkle_tree.cpp:93-113) of each
+ * leaf: paths_out holds count x log2(target_len) digests. */
+int tvh_server_paths(tvh_server* server, uint64_t job_id, const size_t* leaf_indices,
+                     size_t count, uint64_t* paths_out);
+/* Device pointers of a finished job's codewords (column-major) and tree
+ * (level order), valid until the job is released. */
+int tvh_server_buffers(tvh_server* server, uint64_t job_id, const uint64_t** d_codewords,
+                       const uint64_t** d_tree);
+/* Frees a finished or failed job's memory. */
+int tvh_server_release(tvh_server* server, uint64_t job_id);
+int tvh_server_stats(tvh_server* server, size_t* arena_used, size_t* arena_size,
+                     size_t* queued, size_t* running);
+/* Waits for running jobs, fails queued ones, frees everything. */
+int tvh_server_destroy(tvh_server* server);
+
 /* ---- host-pointer forms (synchronous; CudaBackend-style) ------------------ */
 
 /* Replace CudaBackend::ntt_forward / ntt_inverse (src/gpu/cuda_ntt.cu:13-50). */

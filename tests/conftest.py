@@ -73,5 +73,32 @@ def tvh(tvhmod):
     return lib
 
 
+@pytest.fixture(scope="session")
+def T(tvh, tvhmod):
+    hip_rt = tvhmod.hip_rt
+
+    class Ctx:
+        lib = tvh
+        mod = tvhmod
+
+        @staticmethod
+        def dev(a):
+            return tvhmod.to_device(a)
+
+        @staticmethod
+        def host(buf, n=None):
+            return buf.download(n)
+
+        @staticmethod
+        def empty(n):
+            return tvhmod.empty(n)
+
+        @staticmethod
+        def sync():
+            hip_rt.synchronize()
+
+    return Ctx
+
+
 def u64(a):
     return np.ascontiguousarray(a, dtype=np.uint64)

@@ -20,33 +20,6 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype="<u8").tobytes()).hexdigest()
 
 
-@pytest.fixture(scope="module")
-def T(tvh, tvhmod):
-    hip_rt = tvhmod.hip_rt
-
-    class Ctx:
-        lib = tvh
-        mod = tvhmod
-
-        @staticmethod
-        def dev(a):
-            return tvhmod.to_device(a)
-
-        @staticmethod
-        def host(buf, n=None):
-            return buf.download(n)
-
-        @staticmethod
-        def empty(n):
-            return tvhmod.empty(n)
-
-        @staticmethod
-        def sync():
-            hip_rt.synchronize()
-
-    return Ctx
-
-
 # ------------------------------------------------------------------ Tip5
 def test_permutation_golden_and_random(T, oracle, golden):
     states = [c["in"] for c in golden["tip5_permutation"]]
@@ -303,3 +276,31 @@ def test_commit_full_size_sampled(T, oracle):
             out = T.empty(5 * len(pidx))
             T.lib.merkle_gather(tree, m, 0, T.dev(pidx), len(pidx), out)
             assert oracle.verify_path(root, r, leaf, T.host(out).reshape(-1, 5))
+
+
+# ------------------------------------------------------------------ streaming (frugal) commit
+@pytest.mark.parametrize("w,chunk,k", [(1, 1, 0), (9, 4, 0), (10, 3, 0), (23, 7, 2), (37, 10, 0),
+                                       (64, 32, 0), (379, 32, 5), (21, 100, 0)])
+def test_commit_frugal_matches_plain(T, oracle, w, chunk, k):
+    n, m = 64, 256
+    t = oracle.table_rowmajor_draw(40 + w, n, w)
+    r = oracle.splitmix(41 + w, w * k) if k else None
+    cw = T.empty(m * w)
+    tree = T.empty(5 * (2 * m - 1))
+    T.lib.commit(T.dev(t), w, n, T.dev(r) if k else None, k, 1, 7, m, cw, tree, 0)
+    work = T.empty(T.lib.frugal_workspace_elems(w, n, m, chunk))
+    tree2 = T.empty(5 * (2 * m - 1))
+    T.lib.commit_frugal(T.dev(t), w, n, T.dev(r) if k else None, k, 1, 7, m, chunk, work, tree2, 0)
+    assert np.array_equal(T.host(tree2), T.host(tree))
+    root, _, _ = oracle.commit(t, n, w, m, rand=None if r is None else r.reshape(w, k))
+    assert np.array_equal(T.host(tree2).reshape(-1, 5)[-1], root)
+
+
+def test_commit_frugal_heap_layout(T, oracle):
+    n, w, m = 32, 19, 128
+    t = oracle.table_rowmajor_draw(5, n, w)
+    work = T.empty(T.lib.frugal_workspace_elems(w, n, m, 8))
+    tree = T.empty(5 * 2 * m)
+    T.lib.commit_frugal(T.dev(t), w, n, None, 0, 1, 7, m, 8, work, tree, 1)
+    root, _, _ = oracle.commit(t, n, w, m)
+    assert np.array_equal(T.host(tree).reshape(-1, 5)[1], root)

@@ -346,6 +346,52 @@ int tvh_commit_dev(const uint64_t* trace, size_t cols, size_t n, const uint64_t*
     return launch_merkle_levels(tree, m, layout, STREAM(s));
 }
 
+int tvh_fill_uniform_dev(uint64_t* d, size_t count, uint64_t seed, void* s) {
+    if (count && !d) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (!current_device()) return TVH_ERR_DEVICE;
+    return launch_fill_uniform(d, count, seed, STREAM(s));
+}
+
+size_t tvh_commit_frugal_workspace_elems(size_t cols, size_t n, size_t m, size_t chunk) {
+    (void)n;
+    if (chunk == 0) return 0;
+    size_t c = chunk < cols ? chunk : cols;
+    // chunk codewords + 16 state words + 9 carried elements per row
+    return m * (c + 16 + 9);
+}
+
+int tvh_commit_frugal_dev(const uint64_t* trace, size_t cols, size_t n, const uint64_t* rand,
+                          size_t k, uint64_t toff, uint64_t goff, size_t m, size_t chunk,
+                          uint64_t* work, uint64_t* tree, int layout, void* s) {
+    if (layout != TVH_TREE_LEVEL_ORDER && layout != TVH_TREE_HEAP)
+        return set_error(TVH_ERR_LAYOUT, "layout %d", layout);
+    if (!work || !tree || (cols && !trace)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (k && !rand) return set_error(TVH_ERR_INVALID_ARG, "null randomizers");
+    if (chunk == 0) return set_error(TVH_ERR_INVALID_ARG, "chunk_cols must be > 0");
+    if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lengths");
+    if (!current_device()) return TVH_ERR_DEVICE;
+    size_t c = chunk < cols ? chunk : cols;
+    u64* cw = work;
+    u64* state = work + m * c;
+    u64* pend = state + m * 16;
+    u64* dig = tree + 5 * merkle_level_base(m, 0, layout);
+    hipStream_t st = STREAM(s);
+    int rc = TVH_OK;
+    if (cols == 0) {
+        rc = launch_absorb_chunk(nullptr, m, 0, state, pend, 0, true, true, dig, st);
+    }
+    for (size_t c0 = 0; c0 < cols && !rc; c0 += c) {
+        size_t nc = cols - c0 < c ? cols - c0 : c;
+        rc = launch_lde(trace + c0 * n, nc, n, bfe_rand(k ? rand + c0 * k : nullptr, k), toff, goff,
+                        m, false, cw, st);
+        if (!rc)
+            rc = launch_absorb_chunk(cw, m, nc, state, pend, (u32)(c0 % 10), c0 == 0,
+                                     c0 + nc == cols, dig, st);
+    }
+    if (rc) return rc;
+    return launch_merkle_levels(tree, m, layout, st);
+}
+
 // ---------------------------------------------------------------- host forms
 struct HostCall {
     hipStream_t st = nullptr;

@@ -152,25 +152,36 @@ __device__ __forceinline__ u64 twiddle(const Roots& rt, int dir, u32 x, u32 logN
     return gl::mul(rt.lo[dir][y & 4095], rt.hi[dir][y >> 12]);
 }
 
+// Per-block segment lookup for multi-table launches: start[i] is the first
+// block of segment i (start[nseg] = total blocks); returns the segment of blk.
+__device__ __forceinline__ u32 find_segment(const u32* __restrict__ start, u32 nseg, u32 blk) {
+    u32 lo = 0, hi = nseg;  // invariant: start[lo] <= blk < start[hi]
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) >> 1;
+        if (start[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------- step 1
 // Transforms s = (c, n2), element n1 at src[c*sstride + N2*n1 + n2]; DFT over n1
 // (inverse if INV) -> k1; times w_N^(+-n2*k1); to dst[c*dstride + E*N2*k1 + n2].
 template <bool INV>
-__global__ void __launch_bounds__(TPB) k_step1(const u64* __restrict__ src, u64 sstride,
-                                               u64* __restrict__ dst, u64 dstride, u32 logN,
-                                               u32 a, u32 E, u64 ntr, u32 logNT, Roots rt) {
+__device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
+    const u32 logN = A.logN, a = A.a, logNT = A.logNT;
     const u32 b = logN - a;
     const int L = 1 << a;
     const int Lp = padded_len(L);
     const int NT = 1 << logNT;
-    const u64 s0 = (u64)blockIdx.x << logNT;
+    const u64 s0 = blk << logNT;
     for (int e = threadIdx.x; e < NT * L; e += TPB) {
         int t = e & (NT - 1), i = e >> logNT;
         u64 s = s0 + t;
-        if (s < ntr) {
+        if (s < A.ntr) {
             u64 c = s >> b, n2 = s & ((1ull << b) - 1);
-            lds[pidx(t, i, Lp)] = src[c * sstride + ((u64)i << b) + n2];
+            lds[pidx(t, i, Lp)] = A.src[c * A.sstride + ((u64)i << b) + n2];
         }
     }
     __syncthreads();
@@ -178,13 +189,27 @@ __global__ void __launch_bounds__(TPB) k_step1(const u64* __restrict__ src, u64 
     for (int e = threadIdx.x; e < NT * L; e += TPB) {
         int t = e & (NT - 1), k1 = e >> logNT;
         u64 s = s0 + t;
-        if (s < ntr) {
+        if (s < A.ntr) {
             u64 c = s >> b, n2 = s & ((1ull << b) - 1);
             u64 v = lds[pidx(t, k1, Lp)];
             if (n2 && k1) v = gl::mul(v, twiddle(rt, INV, (u32)(n2 * k1), logN));
-            dst[c * dstride + (((u64)E * k1) << b) + n2] = v;
+            A.dst[c * A.dstride + (((u64)A.E * k1) << b) + n2] = v;
         }
     }
+}
+
+template <bool INV>
+__global__ void __launch_bounds__(TPB) k_step1(Step1Args A, Roots rt) {
+    step1_body<INV>(A, blockIdx.x, rt);
+}
+
+template <bool INV>
+__global__ void __launch_bounds__(TPB) k_step1_multi(const Step1Args* __restrict__ args,
+                                                     const u32* __restrict__ start, u32 nseg,
+                                                     Roots rt) {
+    u32 seg = find_segment(start, nseg, blockIdx.x);
+    Step1Args A = args[seg];
+    step1_body<INV>(A, blockIdx.x - start[seg], rt);
 }
 
 // ---------------------------------------------------------------- plain NTT step 2
@@ -220,18 +245,6 @@ __global__ void __launch_bounds__(TPB) k_ntt_step2(const u64* __restrict__ src,
 }
 
 // ---------------------------------------------------------------- LDE step 2
-struct LdeArgs {
-    u32 logN, a, E, logE;
-    u64 cols, M;
-    const u64* src;        // rows of K2: src[c*src_col_stride + k1*src_row_stride + n2]
-    u64 src_col_stride, src_row_stride;
-    const u64* pa;     // pa[s*N1 + k1] = N^-1 * beta_s^k1
-    const u64* pb;     // pb[s*N2 + k2] = beta_s^(N1*k2)
-    u64 delta_n[16];   // (g^N w_E^s - o^N) * N  (or -o^N * N when truncating with E = 1)
-    u64 toff;          // o
-    RandSpec rs;
-};
-
 __device__ __forceinline__ u64 rand_term(const LdeArgs& A, u64 c, u32 s, u64 r) {
     const RandSpec& rs = A.rs;
     u32 limb = (u32)(c % rs.group);
@@ -241,15 +254,15 @@ __device__ __forceinline__ u64 rand_term(const LdeArgs& A, u64 c, u32 s, u64 r) 
     return gl::mul(gl::mul(*p, A.delta_n[s]), gl::pow(A.toff, r));
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_step2(u64* out, LdeArgs A, u32 logNT,
-                                                   Roots rt) {
+__device__ __forceinline__ void lde_step2_body(const LdeArgs& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
-    const u32 a = A.a, b = A.logN - A.a, E = A.E;
+    u64* out = A.out;
+    const u32 a = A.a, b = A.logN - A.a, E = A.E, logNT = A.logNT;
     const int L = 1 << b;
     const int Lp = padded_len(L);
     const int NT = 1 << logNT;           // rows per block
     const u64 nrows = A.cols << a;
-    const u64 s0 = (u64)blockIdx.x << logNT;
+    const u64 s0 = blk << logNT;
     // load Y rows into transforms 0..NT-1 (slot s = 0)
     for (int e = threadIdx.x; e < NT * L; e += TPB) {
         int t = e >> b, i = e & (L - 1);
@@ -295,26 +308,37 @@ __global__ void __launch_bounds__(TPB) k_lde_step2(u64* out, LdeArgs A, u32 logN
     }
 }
 
+__global__ void __launch_bounds__(TPB) k_lde_step2(LdeArgs A, Roots rt) {
+    lde_step2_body(A, blockIdx.x, rt);
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_step2_multi(const LdeArgs* __restrict__ args,
+                                                         const u32* __restrict__ start, u32 nseg,
+                                                         Roots rt) {
+    u32 seg = find_segment(start, nseg, blockIdx.x);
+    const LdeArgs& A = args[seg];  // read in place: a private copy would spill delta_n
+    lde_step2_body(A, blockIdx.x - start[seg], rt);
+}
+
 // ---------------------------------------------------------------- LDE step 3
 // Transforms (c, q = u1*E + s), element k1 at out[c*M + E*N2*k1 + q]; forward
 // DFT over k1 -> u2, written back to out[c*M + E*N2*u2 + q].
-__global__ void __launch_bounds__(TPB) k_lde_step3(u64* __restrict__ out, u32 logN, u32 a,
-                                                   u32 logE, u64 cols, u64 M, u32 logNT,
-                                                   Roots rt) {
+__device__ __forceinline__ void lde_step3_body(const Step3Args& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
-    const u32 b = logN - a;
+    u64* out = A.out;
+    const u32 b = A.logN - A.a, a = A.a, logNT = A.logNT;
     const int L = 1 << a;
     const int Lp = padded_len(L);
     const int NT = 1 << logNT;
-    const u32 logQ = b + logE;  // transforms per column = E*N2
-    const u64 ntr = cols << logQ;
-    const u64 s0 = (u64)blockIdx.x << logNT;
+    const u32 logQ = b + A.logE;  // transforms per column = E*N2
+    const u64 ntr = A.cols << logQ;
+    const u64 s0 = blk << logNT;
     for (int e = threadIdx.x; e < NT * L; e += TPB) {
         int t = e & (NT - 1), i = e >> logNT;
         u64 s = s0 + t;
         if (s < ntr) {
             u64 c = s >> logQ, q = s & ((1ull << logQ) - 1);
-            lds[pidx(t, i, Lp)] = out[c * M + ((u64)i << logQ) + q];
+            lds[pidx(t, i, Lp)] = out[c * A.M + ((u64)i << logQ) + q];
         }
     }
     __syncthreads();
@@ -324,9 +348,21 @@ __global__ void __launch_bounds__(TPB) k_lde_step3(u64* __restrict__ out, u32 lo
         u64 s = s0 + t;
         if (s < ntr) {
             u64 c = s >> logQ, q = s & ((1ull << logQ) - 1);
-            out[c * M + ((u64)u2 << logQ) + q] = lds[pidx(t, u2, Lp)];
+            out[c * A.M + ((u64)u2 << logQ) + q] = lds[pidx(t, u2, Lp)];
         }
     }
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_step3(Step3Args A, Roots rt) {
+    lde_step3_body(A, blockIdx.x, rt);
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_step3_multi(const Step3Args* __restrict__ args,
+                                                         const u32* __restrict__ start, u32 nseg,
+                                                         Roots rt) {
+    u32 seg = find_segment(start, nseg, blockIdx.x);
+    Step3Args A = args[seg];
+    lde_step3_body(A, blockIdx.x - start[seg], rt);
 }
 
 // Constant tables pa/pb of one (logN, a, E, o, g).
@@ -341,6 +377,24 @@ __global__ void k_lde_consts(u64* pa, u64* pb, u32 a, u32 b, u32 E, const u64* b
         u64 s = j >> b, k2 = j & (n2 - 1);
         pb[j] = gl::pow(gl::pow(beta[s], n1), k2);
     }
+}
+
+// Synthetic uniform inputs: counter-based SplitMix64 finaliser of seed + (i+1)*gamma.
+__global__ void k_fill_uniform(u64* d, u64 count, u64 seed) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    u64 z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    d[i] = z >= gl::P ? z - gl::P : z;
+}
+
+int launch_fill_uniform(u64* d, u64 count, u64 seed, hipStream_t st) {
+    if (!count) return TVH_OK;
+    hipLaunchKernelGGL(k_fill_uniform, dim3((u32)((count + 255) / 256)), dim3(256), 0, st, d,
+                       count, seed);
+    return check_launch();
 }
 
 // XFE interleaved <-> limb-major.  Interleaved: (c*len + r)*3 + k; limb-major: (3c + k)*len + r.
@@ -443,8 +497,10 @@ static int lde_consts(DeviceState* ds, u32 logN, u32 a, u32 E, u64 toff, u64 gof
     return TVH_OK;
 }
 
-int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff, u64 m,
-               bool truncate, u64* out, hipStream_t st) {
+int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
+                u64 m, bool truncate, u64* out, LdePlan* P) {
+    P->k1 = P->k3 = false;
+    P->s1_blocks = P->s2_blocks = P->s3_blocks = 0;
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
     if (m < n) return set_error(TVH_ERR_INVALID_ARG, "target_len < trace_len");
@@ -466,18 +522,18 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
 
     // K1: INTT step 1 (transforms (c, n2)); a size-1 step is the identity
     // map trace -> Y, so K2 then reads the trace itself.
-    const bool k1_needed = a > 0;
-    if (k1_needed) {
+    P->k1 = a > 0;
+    if (P->k1) {
         u32 logNT = tile_log_nt(a, 30);
         u64 ntr = cols << b;
-        u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
-        hipLaunchKernelGGL(k_step1<true>, dim3((u32)blocks), dim3(TPB), lds_bytes(a, logNT), st,
-                           trace, n, out, m, logN, a, E, ntr, logNT, ds->roots);
-        if ((rc = check_launch())) return rc;
+        P->s1 = Step1Args{trace, n, out, m, logN, a, E, ntr, logNT};
+        P->s1_blocks = (u32)((ntr + (1ull << logNT) - 1) >> logNT);
+        P->s1_lds = lds_bytes(a, logNT);
     }
     // K2: INTT step 2 + coset scaling + forward step 1
     {
-        LdeArgs A;
+        LdeArgs& A = P->s2;
+        A.out = out;
         A.logN = logN;
         A.a = a;
         A.E = E;
@@ -500,24 +556,61 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
         if (b < 5) logNT = LOG_TILE - 5 - logE;
         u64 nrows = cols << a;
         if ((1ull << logNT) > nrows) logNT = log2u(nrows);
-        u64 blocks = (nrows + (1ull << logNT) - 1) >> logNT;
-        A.src = k1_needed ? out : trace;
-        A.src_col_stride = k1_needed ? m : n;
-        A.src_row_stride = k1_needed ? ((u64)E << b) : 0;
-        hipLaunchKernelGGL(k_lde_step2, dim3((u32)blocks), dim3(TPB), lds_bytes(b, logNT + logE),
-                           st, out, A, logNT, ds->roots);
-        if ((rc = check_launch())) return rc;
+        A.logNT = logNT;
+        A.src = P->k1 ? out : trace;
+        A.src_col_stride = P->k1 ? m : n;
+        A.src_row_stride = P->k1 ? ((u64)E << b) : 0;
+        P->s2_blocks = (u32)((nrows + (1ull << logNT) - 1) >> logNT);
+        P->s2_lds = lds_bytes(b, logNT + logE);
     }
     // K3: forward step 2 for all cosets (identity when N1 = 1)
-    if (a > 0) {
+    P->k3 = a > 0;
+    if (P->k3) {
         u32 logNT = tile_log_nt(a, 30);
         u64 ntr = cols << (b + logE);
-        u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
-        hipLaunchKernelGGL(k_lde_step3, dim3((u32)blocks), dim3(TPB), lds_bytes(a, logNT), st,
-                           out, logN, a, logE, cols, m, logNT, ds->roots);
+        P->s3 = Step3Args{out, logN, a, logE, cols, m, logNT};
+        P->s3_blocks = (u32)((ntr + (1ull << logNT) - 1) >> logNT);
+        P->s3_lds = lds_bytes(a, logNT);
+    }
+    return TVH_OK;
+}
+
+int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff, u64 m,
+               bool truncate, u64* out, hipStream_t st) {
+    LdePlan P;
+    int rc = prepare_lde(trace, cols, n, rs, toff, goff, m, truncate, out, &P);
+    if (rc || cols == 0) return rc;
+    DeviceState* ds = current_device();
+    if (P.k1) {
+        hipLaunchKernelGGL(k_step1<true>, dim3(P.s1_blocks), dim3(TPB), P.s1_lds, st, P.s1,
+                           ds->roots);
+        if ((rc = check_launch())) return rc;
+    }
+    hipLaunchKernelGGL(k_lde_step2, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2, ds->roots);
+    if ((rc = check_launch())) return rc;
+    if (P.k3) {
+        hipLaunchKernelGGL(k_lde_step3, dim3(P.s3_blocks), dim3(TPB), P.s3_lds, st, P.s3,
+                           ds->roots);
         if ((rc = check_launch())) return rc;
     }
     return TVH_OK;
+}
+
+int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg, u32 blocks,
+                     size_t lds, hipStream_t st) {
+    if (blocks == 0) return TVH_OK;
+    DeviceState* ds = current_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    if (phase == 1)
+        hipLaunchKernelGGL(k_step1_multi<true>, dim3(blocks), dim3(TPB), lds, st,
+                           (const Step1Args*)d_args, d_start, nseg, ds->roots);
+    else if (phase == 2)
+        hipLaunchKernelGGL(k_lde_step2_multi, dim3(blocks), dim3(TPB), lds, st,
+                           (const LdeArgs*)d_args, d_start, nseg, ds->roots);
+    else
+        hipLaunchKernelGGL(k_lde_step3_multi, dim3(blocks), dim3(TPB), lds, st,
+                           (const Step3Args*)d_args, d_start, nseg, ds->roots);
+    return check_launch();
 }
 
 int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st) {
@@ -548,14 +641,13 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st)
             u32 logNT = tile_log_nt(a, 30);
             u64 ntr = g << b;
             u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
+            Step1Args A{d, n, scratch, n, logN, a, 1u, ntr, logNT};
             if (inverse)
                 hipLaunchKernelGGL(k_step1<true>, dim3((u32)blocks), dim3(TPB),
-                                   lds_bytes(a, logNT), st, d, n, scratch, n, logN, a, 1u, ntr,
-                                   logNT, ds->roots);
+                                   lds_bytes(a, logNT), st, A, ds->roots);
             else
                 hipLaunchKernelGGL(k_step1<false>, dim3((u32)blocks), dim3(TPB),
-                                   lds_bytes(a, logNT), st, d, n, scratch, n, logN, a, 1u, ntr,
-                                   logNT, ds->roots);
+                                   lds_bytes(a, logNT), st, A, ds->roots);
             if ((rc = check_launch())) return rc;
         }
         {
@@ -581,7 +673,9 @@ int ntt_set_attributes() {
     const int bytes = 150 * 1024;
     const void* fns[] = {(const void*)k_step1<true>, (const void*)k_step1<false>,
                          (const void*)k_ntt_step2<true>, (const void*)k_ntt_step2<false>,
-                         (const void*)k_lde_step2, (const void*)k_lde_step3};
+                         (const void*)k_lde_step2, (const void*)k_lde_step3,
+                         (const void*)k_step1_multi<true>, (const void*)k_lde_step2_multi,
+                         (const void*)k_lde_step3_multi};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
             return set_error(TVH_ERR_DEVICE, "LDS attribute");

@@ -25,18 +25,27 @@ __device__ __forceinline__ void store_digest(u64* dst, const u64 s[tip5::STATE])
     for (int i = 0; i < tip5::DIGEST; i++) dst[i] = gl::canon(s[i]);
 }
 
+// Segment lookup for multi-table launches (start[nseg] = total blocks).
+__device__ __forceinline__ u32 find_seg(const u32* __restrict__ start, u32 nseg, u32 blk) {
+    u32 lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        u32 mid = (lo + hi) >> 1;
+        if (start[mid] <= blk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // rows x cols column-major table -> rows digests (5 u64 each, row-major).
-__global__ void __launch_bounds__(256) k_hash_rows(const u64* __restrict__ tab, u64 rows,
-                                                   u32 cols, u64* __restrict__ dig) {
-    __shared__ uint8_t lut[256];
-    tip5::load_lut(lut);
-    __syncthreads();
-    u64 row = (u64)blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= rows) return;
+__device__ __forceinline__ void hash_rows_body(const HashArgs& A, u64 blk, const uint8_t* lut) {
+    u64 row = blk * 256 + threadIdx.x;
+    if (row >= A.rows) return;
+    const u64 rows = A.rows;
+    const u32 cols = A.cols;
     u64 s[tip5::STATE];
 #pragma unroll
     for (int i = 0; i < tip5::STATE; i++) s[i] = 0;
-    const u64* p = tab + row;
+    const u64* p = A.tab + row;
     u32 used = 0;
     for (; used + tip5::RATE <= cols; used += tip5::RATE) {
 #pragma unroll
@@ -51,7 +60,77 @@ __global__ void __launch_bounds__(256) k_hash_rows(const u64* __restrict__ tab, 
                          : (ui == rest ? 1ull : 0ull);
     }
     tip5::permutation(s, lut);
-    store_digest(dig + row * tip5::DIGEST, s);
+    store_digest(A.dig + row * tip5::DIGEST, s);
+}
+
+__global__ void __launch_bounds__(256) k_hash_rows(HashArgs A) {
+    __shared__ uint8_t lut[256];
+    tip5::load_lut(lut);
+    __syncthreads();
+    hash_rows_body(A, blockIdx.x, lut);
+}
+
+__global__ void __launch_bounds__(256) k_hash_rows_multi(const HashArgs* __restrict__ args,
+                                                         const u32* __restrict__ start, u32 nseg) {
+    __shared__ uint8_t lut[256];
+    tip5::load_lut(lut);
+    __syncthreads();
+    u32 seg = find_seg(start, nseg, blockIdx.x);
+    HashArgs A = args[seg];
+    hash_rows_body(A, blockIdx.x - start[seg], lut);
+}
+
+// Column-chunk absorption for the streaming ("frugal") commit: each row's
+// hash_varlen sponge continues across chunks of columns.  The row's element
+// stream is the npend elements left over from the previous chunks (pend,
+// column-major by position) followed by this chunk's ccols columns; full rate
+// blocks are absorbed by overwrite, the remainder is carried.  All rows have
+// the same counts, so lanes never diverge.  State is column-major by word
+// (state[i*rows + row]).  The FINAL chunk absorbs the padding block
+// [rest, 1, 0...] and writes the canonical digest (tip5.cpp:246-282).
+template <bool FINAL>
+__global__ void __launch_bounds__(256) k_absorb_chunk(const u64* __restrict__ chunk, u64 rows,
+                                                      u32 ccols, u64* __restrict__ state,
+                                                      u64* __restrict__ pend, u32 npend,
+                                                      int first, u64* __restrict__ dig) {
+    __shared__ uint8_t lut[256];
+    tip5::load_lut(lut);
+    __syncthreads();
+    u64 row = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    u64 s[tip5::STATE];
+#pragma unroll
+    for (int i = 0; i < tip5::STATE; i++) s[i] = first ? 0ull : state[(u64)i * rows + row];
+    auto elem = [&](u32 j) -> u64 {
+        return j < npend ? pend[(u64)j * rows + row] : chunk[(u64)(j - npend) * rows + row];
+    };
+    const u32 total = npend + ccols;
+    u32 used = 0;
+    for (; used + tip5::RATE <= total; used += tip5::RATE) {
+#pragma unroll
+        for (int i = 0; i < tip5::RATE; i++) s[i] = elem(used + i);
+        tip5::permutation(s, lut);
+    }
+    const u32 rest = total - used;
+    if (FINAL) {
+#pragma unroll
+        for (int i = 0; i < tip5::RATE; i++) {
+            u32 ui = (u32)i;
+            s[i] = ui < rest ? elem(used + ui) : (ui == rest ? 1ull : 0ull);
+        }
+        tip5::permutation(s, lut);
+        store_digest(dig + row * tip5::DIGEST, s);
+    } else {
+        // carried elements first (they may be read from pend itself: read all, then write)
+        u64 carry[tip5::RATE];
+#pragma unroll
+        for (int i = 0; i < tip5::RATE; i++) carry[i] = (u32)i < rest ? elem(used + i) : 0ull;
+#pragma unroll
+        for (int i = 0; i < tip5::RATE; i++)
+            if ((u32)i < rest) pend[(u64)i * rows + row] = carry[i];
+#pragma unroll
+        for (int i = 0; i < tip5::STATE; i++) state[(u64)i * rows + row] = s[i];
+    }
 }
 
 // Tip5 permutation of num states (16 u64 each, in place); outputs canonical.
@@ -95,38 +174,29 @@ __global__ void __launch_bounds__(256) k_hash_pairs(const u64* __restrict__ left
     store_digest(out + 5 * i, s);
 }
 
-// One launch of the Merkle build: children are the nodes of level `lvl` (count
-// `nchild`, starting at `src`); each block of T threads hashes 2T children into
-// T parents and continues for `levels` levels inside LDS.  Level l+j nodes go
-// to dst_base[j-1] + index.
-struct MerkleLevels {
-    u64* dst[9];
-};
-
-__global__ void __launch_bounds__(256) k_merkle_subtrees(const u64* __restrict__ src,
-                                                         MerkleLevels out, u32 levels) {
-    __shared__ uint8_t lut[256];
-    __shared__ u64 buf[256 * 5];
-    tip5::load_lut(lut);
-    __syncthreads();
-    const u32 T = blockDim.x;
+// One launch of the Merkle build: children are the nodes of one level; each
+// block hashes 2T children into T parents (T active threads, T <= blockDim)
+// and continues for `levels` levels inside LDS.  Level j (1-based) parents go
+// to dst[j-1] + index.
+__device__ __forceinline__ void merkle_body(const MerkleArgs& A, u64 blk, const uint8_t* lut,
+                                            u64* buf) {
+    const u32 T = A.T;
     const u32 t = threadIdx.x;
     u64 s[tip5::STATE];
-    // first level: children from global memory
-    {
-        u64 parent = (u64)blockIdx.x * T + t;
-        hash_pair(src + 10 * parent, src + 10 * parent + 5, s, lut);
+    if (t < T) {
+        u64 parent = blk * T + t;
+        hash_pair(A.src + 10 * parent, A.src + 10 * parent + 5, s, lut);
         u64 d[5];
 #pragma unroll
         for (int k = 0; k < 5; k++) d[k] = gl::canon(s[k]);
 #pragma unroll
         for (int k = 0; k < 5; k++) {
-            out.dst[0][5 * parent + k] = d[k];
+            A.dst[0][5 * parent + k] = d[k];
             buf[5 * t + k] = d[k];
         }
     }
     u32 width = T;
-    for (u32 j = 1; j < levels; j++) {
+    for (u32 j = 1; j < A.levels; j++) {
         __syncthreads();
         width >>= 1;
         u64 d[5];
@@ -137,14 +207,34 @@ __global__ void __launch_bounds__(256) k_merkle_subtrees(const u64* __restrict__
         }
         __syncthreads();
         if (t < width) {
-            u64 parent = (u64)blockIdx.x * width + t;
+            u64 parent = blk * width + t;
 #pragma unroll
             for (int k = 0; k < 5; k++) {
-                out.dst[j][5 * parent + k] = d[k];
+                A.dst[j][5 * parent + k] = d[k];
                 buf[5 * t + k] = d[k];
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(256) k_merkle_subtrees(MerkleArgs A) {
+    __shared__ uint8_t lut[256];
+    __shared__ u64 buf[256 * 5];
+    tip5::load_lut(lut);
+    __syncthreads();
+    merkle_body(A, blockIdx.x, lut, buf);
+}
+
+__global__ void __launch_bounds__(256) k_merkle_subtrees_multi(const MerkleArgs* __restrict__ args,
+                                                               const u32* __restrict__ start,
+                                                               u32 nseg) {
+    __shared__ uint8_t lut[256];
+    __shared__ u64 buf[256 * 5];
+    tip5::load_lut(lut);
+    __syncthreads();
+    u32 seg = find_seg(start, nseg, blockIdx.x);
+    const MerkleArgs& A = args[seg];  // read in place: a private copy would spill dst[]
+    merkle_body(A, blockIdx.x - start[seg], lut, buf);
 }
 
 // Gather: out[q] = node with heap index idx[q] (1 <= idx < 2n) of a tree in
@@ -174,7 +264,20 @@ int launch_hash_rows(const u64* tab, u64 rows, u64 cols, u64* dig, hipStream_t s
     if (rows == 0) return TVH_OK;
     if (cols > 0xFFFFFFFFull) return TVH_ERR_INVALID_ARG;
     u32 blocks = (u32)((rows + 255) / 256);
-    hipLaunchKernelGGL(k_hash_rows, dim3(blocks), dim3(256), 0, st, tab, rows, (u32)cols, dig);
+    hipLaunchKernelGGL(k_hash_rows, dim3(blocks), dim3(256), 0, st, HashArgs{tab, rows, (u32)cols, dig});
+    return check_launch();
+}
+
+int launch_absorb_chunk(const u64* chunk, u64 rows, u64 ccols, u64* state, u64* pend, u32 npend,
+                        bool first, bool final_chunk, u64* dig, hipStream_t st) {
+    if (rows == 0) return TVH_OK;
+    u32 blocks = (u32)((rows + 255) / 256);
+    if (final_chunk)
+        hipLaunchKernelGGL(k_absorb_chunk<true>, dim3(blocks), dim3(256), 0, st, chunk, rows,
+                           (u32)ccols, state, pend, npend, (int)first, dig);
+    else
+        hipLaunchKernelGGL(k_absorb_chunk<false>, dim3(blocks), dim3(256), 0, st, chunk, rows,
+                           (u32)ccols, state, pend, npend, (int)first, dig);
     return check_launch();
 }
 
@@ -196,10 +299,11 @@ u64 merkle_level_base(u64 n, u32 level, int layout) {
     return layout == TVH_TREE_HEAP ? (n >> level) : (2 * n - ((2 * n) >> level));
 }
 
-// Build levels 1..log2(n) of a tree whose leaves are already at level 0.
-int launch_merkle_levels(u64* tree, u64 n, int layout, hipStream_t st) {
-    u32 logn = 0;
-    while ((1ull << logn) < n) logn++;
+// Launch groups of the Merkle build of an n-leaf tree whose leaves are already
+// at level 0: group g covers up to 9 levels; T active threads per block.
+int merkle_groups(u64* tree, u64 n, int layout, std::vector<MerkleGroup>& groups) {
+    groups.clear();
+    u32 logn = log2u(n);
     u32 lvl = 0;
     while (lvl < logn) {
         u64 nchild = n >> lvl;
@@ -208,17 +312,44 @@ int launch_merkle_levels(u64* tree, u64 n, int layout, hipStream_t st) {
         u32 maxlev = 0;
         while ((1u << maxlev) <= T) maxlev++;           // T=256 -> 9 levels
         u32 levels = logn - lvl < maxlev ? logn - lvl : maxlev;
-        MerkleLevels ml;
+        MerkleGroup g;
+        g.args.src = tree + 5 * merkle_level_base(n, lvl, layout);
         for (u32 j = 0; j < 9; j++)
-            ml.dst[j] = j < levels ? tree + 5 * merkle_level_base(n, lvl + j + 1, layout) : nullptr;
-        const u64* src = tree + 5 * merkle_level_base(n, lvl, layout);
-        hipLaunchKernelGGL(k_merkle_subtrees, dim3((u32)(parents / T)), dim3(T), 0, st, src, ml,
-                           levels);
-        int rc = check_launch();
-        if (rc) return rc;
+            g.args.dst[j] = j < levels ? tree + 5 * merkle_level_base(n, lvl + j + 1, layout) : nullptr;
+        g.args.levels = levels;
+        g.args.T = T;
+        g.blocks = (u32)(parents / T);
+        groups.push_back(g);
         lvl += levels;
     }
     return TVH_OK;
+}
+
+// Build levels 1..log2(n) of a tree whose leaves are already at level 0.
+int launch_merkle_levels(u64* tree, u64 n, int layout, hipStream_t st) {
+    std::vector<MerkleGroup> groups;
+    merkle_groups(tree, n, layout, groups);
+    for (const MerkleGroup& g : groups) {
+        hipLaunchKernelGGL(k_merkle_subtrees, dim3(g.blocks), dim3(g.args.T), 0, st, g.args);
+        int rc = check_launch();
+        if (rc) return rc;
+    }
+    return TVH_OK;
+}
+
+int launch_hash_rows_multi(const HashArgs* d_args, const u32* d_start, u32 nseg, u32 blocks,
+                           hipStream_t st) {
+    if (!blocks) return TVH_OK;
+    hipLaunchKernelGGL(k_hash_rows_multi, dim3(blocks), dim3(256), 0, st, d_args, d_start, nseg);
+    return check_launch();
+}
+
+int launch_merkle_multi(const MerkleArgs* d_args, const u32* d_start, u32 nseg, u32 blocks,
+                        hipStream_t st) {
+    if (!blocks) return TVH_OK;
+    hipLaunchKernelGGL(k_merkle_subtrees_multi, dim3(blocks), dim3(256), 0, st, d_args, d_start,
+                       nseg);
+    return check_launch();
 }
 
 int launch_gather_heap(const u64* tree, u64 n, int layout, const u64* idx, u64 count, u64* out,

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "../../include/tvh.h"
 #include "gl64.hpp"
@@ -76,12 +77,80 @@ struct RandSpec {
 };
 int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff, u64 m,
                bool truncate, u64* out, hipStream_t st);
+
+// Kernel arguments of the three LDE steps (one table).  Multi-table launches
+// take device arrays of these plus per-segment block offsets.
+struct Step1Args {
+    const u64* src;
+    u64 sstride;
+    u64* dst;
+    u64 dstride;
+    u32 logN, a, E;
+    u64 ntr;
+    u32 logNT;
+};
+struct Step3Args {
+    u64* out;
+    u32 logN, a, logE;
+    u64 cols, M;
+    u32 logNT;
+};
+struct LdeArgs {
+    u64* out;
+    u32 logN, a, E, logE;
+    u64 cols, M;
+    const u64* src;    // rows of step 2: src[c*src_col_stride + k1*src_row_stride + n2]
+    u64 src_col_stride, src_row_stride;
+    const u64* pa;     // pa[s*N1 + k1] = N^-1 * beta_s^k1
+    const u64* pb;     // pb[s*N2 + k2] = beta_s^(N1*k2)
+    u64 delta_n[16];   // (g^N w_E^s - o^N) * N  (or -o^N * N when truncating with E = 1)
+    u64 toff;          // o
+    RandSpec rs;
+    u32 logNT;
+};
+struct LdePlan {
+    bool k1, k3;
+    Step1Args s1;
+    LdeArgs s2;
+    Step3Args s3;
+    u32 s1_blocks, s2_blocks, s3_blocks;
+    size_t s1_lds, s2_lds, s3_lds;
+};
+int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
+                u64 m, bool truncate, u64* out, LdePlan* plan);
+// phase 1/2/3: k_step1_multi<true> / k_lde_step2_multi / k_lde_step3_multi
+int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg, u32 blocks,
+                     size_t lds, hipStream_t st);
+int launch_fill_uniform(u64* d, u64 count, u64 seed, hipStream_t st);
 int launch_xfe_interleave(const u64* src, u64* dst, u64 cols, u64 len, bool to_limb_major,
                           hipStream_t st);
 
 // ---- Tip5 launchers (tip5_kernels.hip) ----
+struct HashArgs {
+    const u64* tab;
+    u64 rows;
+    u32 cols;
+    u64* dig;
+};
+struct MerkleArgs {
+    const u64* src;  // children level
+    u64* dst[9];     // parents of the next `levels` levels
+    u32 levels;
+    u32 T;           // active threads per block (parents per block at the first level)
+};
+struct MerkleGroup {
+    MerkleArgs args;
+    u32 blocks;
+};
+int merkle_groups(u64* tree, u64 n, int layout, std::vector<MerkleGroup>& groups);
+int launch_hash_rows_multi(const HashArgs* d_args, const u32* d_start, u32 nseg, u32 blocks,
+                           hipStream_t st);
+int launch_merkle_multi(const MerkleArgs* d_args, const u32* d_start, u32 nseg, u32 blocks,
+                        hipStream_t st);
 int launch_hash_rows(const u64* tab, u64 rows, u64 cols, u64* dig, hipStream_t st);
 int launch_permutation(u64* states, u64 num, hipStream_t st);
+int launch_absorb_chunk(const u64* chunk, u64 rows, u64 ccols, u64* state, u64* pend, u32 npend,
+                        bool first, bool final_chunk, u64* dig, hipStream_t st);
 int launch_hash_pairs(const u64* l, const u64* r, u64* out, u64 count, hipStream_t st);
 u64 merkle_level_base(u64 n, u32 level, int layout);
 int launch_merkle_levels(u64* tree, u64 n, int layout, hipStream_t st);

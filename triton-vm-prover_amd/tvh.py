@@ -13,6 +13,8 @@ import os
 import re
 import sys
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 if HERE not in sys.path:
     sys.path.insert(0, HERE)
@@ -35,10 +37,39 @@ class TVHError(RuntimeError):
 def declared_symbols(header: str = HEADER) -> list[str]:
     """Names of every entry point declared in include/tvh.h."""
     with open(header) as f:
-        return re.findall(r"^\s*(?:int|const char\*)\s+(tvh_\w+)\(", f.read(), re.M)
+        return re.findall(r"^\s*(?:int|size_t|const char\*)\s+(tvh_\w+)\(", f.read(), re.M)
 
 
 _vp, _sz, _u64, _i32 = C.c_void_p, C.c_size_t, C.c_uint64, C.c_int
+
+class TableDesc(C.Structure):
+    """tvh_table_desc (include/tvh.h)."""
+    _fields_ = [("trace", C.c_void_p), ("num_cols", C.c_size_t), ("trace_len", C.c_size_t),
+                ("rand", C.c_void_p), ("rand_len", C.c_size_t), ("trace_offset", C.c_uint64),
+                ("target_offset", C.c_uint64), ("target_len", C.c_size_t),
+                ("codewords", C.c_void_p), ("tree", C.c_void_p), ("layout", C.c_int)]
+
+
+class ServerConfig(C.Structure):
+    """tvh_server_config (include/tvh.h)."""
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_inflight", C.c_int),
+                ("max_queue", C.c_size_t)]
+
+
+class JobDesc(C.Structure):
+    """tvh_job_desc (include/tvh.h)."""
+    _fields_ = [("host_trace", C.c_void_p), ("host_rand", C.c_void_p), ("seed", C.c_uint64),
+                ("num_cols", C.c_size_t), ("trace_len", C.c_size_t), ("rand_len", C.c_size_t),
+                ("trace_offset", C.c_uint64), ("target_offset", C.c_uint64),
+                ("target_len", C.c_size_t)]
+
+
+class JobTimes(C.Structure):
+    """tvh_job_times (include/tvh.h)."""
+    _fields_ = [("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
 _SIGS = {
     "tvh_init": [_i32],
@@ -62,6 +93,24 @@ _SIGS = {
     "tvh_auth_path_indices": [_sz, _sz, _vp, _sz, _vp],
     "tvh_auth_structure_indices": [_sz, _vp, _sz, _vp, _sz, _vp],
     "tvh_commit_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp, _vp, _i32, _vp],
+    "tvh_batch_plan_create": [_vp, _sz, _vp],
+    "tvh_batch_plan_launch": [_vp, _vp],
+    "tvh_batch_plan_launch_graph": [_vp, _vp],
+    "tvh_batch_plan_destroy": [_vp],
+    "tvh_commit_batch_dev": [_vp, _sz, _vp],
+    "tvh_fill_uniform_dev": [_vp, _sz, _u64, _vp],
+    "tvh_commit_frugal_workspace_elems": [_sz, _sz, _sz, _sz],
+    "tvh_commit_frugal_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _sz, _vp, _vp, _i32, _vp],
+    "tvh_server_create": [_vp, _vp],
+    "tvh_server_submit": [_vp, _vp, _vp],
+    "tvh_server_poll": [_vp, _u64, _vp],
+    "tvh_server_wait": [_vp, _u64, _vp, _vp],
+    "tvh_server_open": [_vp, _u64, _vp, _sz, _vp, _vp, _sz, _vp],
+    "tvh_server_paths": [_vp, _u64, _vp, _sz, _vp],
+    "tvh_server_buffers": [_vp, _u64, _vp, _vp],
+    "tvh_server_release": [_vp, _u64],
+    "tvh_server_stats": [_vp, _vp, _vp, _vp, _vp],
+    "tvh_server_destroy": [_vp],
     "tvh_ntt_forward_host": [_vp, _sz],
     "tvh_ntt_inverse_host": [_vp, _sz],
     "tvh_lde_batch_host": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp],
@@ -95,7 +144,8 @@ class TVH:
         for name, args in _SIGS.items():
             f = getattr(self.lib, name)
             f.argtypes = args
-            f.restype = C.c_char_p if name in ("tvh_status_string", "tvh_last_error") else C.c_int
+            f.restype = (C.c_char_p if name in ("tvh_status_string", "tvh_last_error") else
+                         C.c_size_t if name.endswith("_elems") else C.c_int)
 
     # -- plumbing -------------------------------------------------------------
     def status_string(self, rc: int) -> str:
@@ -195,6 +245,110 @@ class TVH:
                layout=TREE_LEVEL_ORDER, stream=None):
         self._call("tvh_commit_dev", _ptr(trace), cols, n, _ptr(rand), k, toff, goff, m,
                    _ptr(codewords), _ptr(tree), layout, self.stream_ptr(stream))
+
+    # -- batches ----------------------------------------------------------------
+    @staticmethod
+    def table_desc(trace, cols, n, m, codewords, tree, rand=None, k=0, toff=1, goff=7,
+                   layout=TREE_LEVEL_ORDER):
+        return TableDesc(_ptr(trace), cols, n, _ptr(rand), k, toff, goff, m, _ptr(codewords),
+                         _ptr(tree), layout)
+
+    def batch_plan(self, descs):
+        arr = (TableDesc * max(1, len(descs)))(*descs)
+        plan = C.c_void_p()
+        self._call("tvh_batch_plan_create", C.cast(arr, C.c_void_p), len(descs), C.byref(plan))
+        return plan
+
+    def batch_launch(self, plan, stream=None):
+        self._call("tvh_batch_plan_launch", plan, self.stream_ptr(stream))
+
+    def batch_launch_graph(self, plan, stream):
+        self._call("tvh_batch_plan_launch_graph", plan, self.stream_ptr(stream))
+
+    def batch_destroy(self, plan):
+        self._call("tvh_batch_plan_destroy", plan)
+
+    def commit_batch(self, descs, stream=None):
+        arr = (TableDesc * max(1, len(descs)))(*descs)
+        self._call("tvh_commit_batch_dev", C.cast(arr, C.c_void_p), len(descs),
+                   self.stream_ptr(stream))
+
+    def fill_uniform(self, d, count, seed, stream=None):
+        self._call("tvh_fill_uniform_dev", _ptr(d), count, seed, self.stream_ptr(stream))
+
+    def commit_frugal(self, trace, cols, n, rand, k, toff, goff, m, chunk, work, tree,
+                      layout=TREE_LEVEL_ORDER, stream=None):
+        self._call("tvh_commit_frugal_dev", _ptr(trace), cols, n, _ptr(rand), k, toff, goff, m,
+                   chunk, _ptr(work), _ptr(tree), layout, self.stream_ptr(stream))
+
+    def frugal_workspace_elems(self, cols, n, m, chunk):
+        return self.lib.tvh_commit_frugal_workspace_elems(cols, n, m, chunk)
+
+    # -- commit server ------------------------------------------------------------
+    def server_create(self, hbm_budget_bytes=0, max_inflight=0, max_queue=0):
+        cfg = ServerConfig(hbm_budget_bytes, max_inflight, max_queue)
+        srv = C.c_void_p()
+        self._call("tvh_server_create", C.byref(cfg), C.byref(srv))
+        return srv
+
+    def server_submit(self, srv, cols, n, m, seed=0, trace=None, rand=None, k=0, toff=1, goff=7):
+        """trace / rand: numpy uint64 (column-major) kept alive by the caller until the
+        job finished, or None for device-drawn inputs from `seed`."""
+        d = JobDesc(_ptr(trace), _ptr(rand), seed, cols, n, k, toff, goff, m)
+        jid = C.c_uint64()
+        self._call("tvh_server_submit", srv, C.byref(d), C.byref(jid))
+        return jid.value
+
+    def server_poll(self, srv, jid):
+        st = C.c_int()
+        self._call("tvh_server_poll", srv, jid, C.byref(st))
+        return st.value
+
+    def server_wait(self, srv, jid):
+        """-> (root as 5 ints, (submit_s, admit_s, done_s))"""
+        root = np.zeros(5, np.uint64)
+        t = JobTimes()
+        self._call("tvh_server_wait", srv, jid, root.ctypes.data, C.byref(t))
+        return [int(v) for v in root], (t.submit_s, t.admit_s, t.done_s)
+
+    def server_open(self, srv, jid, leaves, cols, want_rows=True, want_auth=True):
+        """-> (rows count x cols or None, auth structure digests k x 5 or None)"""
+        li = np.ascontiguousarray(leaves, dtype=np.uint64)
+        cnt = C.c_size_t()
+        lp = li.ctypes.data if li.size else None
+        auth = None
+        if want_auth:
+            self._call("tvh_server_open", srv, jid, lp, li.size, None, None, 0, C.byref(cnt))
+            auth = np.zeros((max(1, cnt.value), 5), np.uint64)
+        rows = np.zeros((max(1, li.size), max(1, cols)), np.uint64) if want_rows else None
+        self._call("tvh_server_open", srv, jid, lp, li.size, _ptr(rows), _ptr(auth),
+                   cnt.value, C.byref(cnt))
+        return (rows[: li.size, :cols] if want_rows else None), (auth[: cnt.value] if want_auth
+                                                                 else None)
+
+    def server_paths(self, srv, jid, leaves, m):
+        li = np.ascontiguousarray(leaves, dtype=np.uint64)
+        h = int(m).bit_length() - 1
+        out = np.zeros((max(1, li.size), max(1, h), 5), np.uint64)
+        self._call("tvh_server_paths", srv, jid, li.ctypes.data if li.size else None, li.size,
+                   out.ctypes.data)
+        return out[: li.size, :h]
+
+    def server_buffers(self, srv, jid):
+        cw, tree = C.c_void_p(), C.c_void_p()
+        self._call("tvh_server_buffers", srv, jid, C.byref(cw), C.byref(tree))
+        return cw.value, tree.value
+
+    def server_release(self, srv, jid):
+        self._call("tvh_server_release", srv, jid)
+
+    def server_stats(self, srv):
+        v = [C.c_size_t() for _ in range(4)]
+        self._call("tvh_server_stats", srv, *[C.byref(x) for x in v])
+        return dict(zip(("arena_used", "arena_size", "queued", "running"), (x.value for x in v)))
+
+    def server_destroy(self, srv):
+        self._call("tvh_server_destroy", srv)
 
     # -- host forms (numpy uint64 arrays) ----------------------------------------
     def ntt_forward_host(self, a):
