@@ -127,6 +127,7 @@ def run_batch(args, mod, lib) -> dict:
         used += b
     if cur:
         waves.append(cur)
+    tuned = autotune(args, lib, [(1 << lg, w, E << lg) for lg, w in shapes])
     stream = hip_rt.Stream()
     t_plain = t_plan = t_graph = 0.0
     cells = 0
@@ -185,6 +186,7 @@ def run_batch(args, mod, lib) -> dict:
         "cells_per_s": {"per_table": cells / (t_plain / 1000), "batch_plan": cells / (t_plan / 1000),
                         "batch_graph": cells / (t_graph / 1000)},
         "speedup_vs_per_table": t_plain / t_graph,
+        "lde_forms": tuned,
         "root_mismatches": mismatches,
     }
 
@@ -305,6 +307,7 @@ def run_serve(args, mod, lib) -> dict:
     stream = hip_rt.Stream()
     jobs = serve_stream(args, args.jobs, seed=2024)
     cells = sum(j["m"] * j["w"] for j in jobs)
+    tuned = autotune(args, lib, [(j["n"], j["w"], j["m"]) for j in jobs])
     # warm-up: one job of every height through both paths (coset tables, code objects)
     warm = [dict(jobs[0], n=1 << lg, m=args.expansion << lg, seed=7, t=0.0, i=-1 - lg,
                  leaves=jobs[0]["leaves"] % np.uint64(args.expansion << lg))
@@ -370,9 +373,25 @@ def run_serve(args, mod, lib) -> dict:
         "plain_per_table": {"cells_per_s": cells / plain_total, "total_s": plain_total,
                             "s_per_job_by_log_height": {str(k.bit_length() - 1): v
                                                         for k, v in plain_s.items()}},
-        "checked_jobs": len(sample), "mismatches_vs_plain": mism,
+        "checked_jobs": len(sample), "mismatches_vs_plain": mism, "lde_forms": tuned,
         "wall_s": t_end - t0,
     }
+
+
+def autotune(args, lib, shapes):
+    """Runs the library's LDE autotuner once per (trace_len, target_len) with the
+    width given; returns {log2 trace_len: {"form": chosen, "ms": {form: ms}}}."""
+    out = {}
+    if args.no_autotune:
+        return out
+    for n, w, m in sorted(set(shapes)):
+        lg = n.bit_length() - 1
+        if lg in out:
+            continue
+        chosen, ms = lib.lde_autotune(w, n, m)
+        name = sys.modules["tvh"].lde_variant_name
+        out[lg] = {"variant": name(chosen), "ms": {name(k): round(v, 3) for k, v in ms.items()}}
+    return out
 
 
 def workload_name(args, n, w, E) -> str:
@@ -415,6 +434,8 @@ def main() -> int:
     ap.add_argument("--saturated-jobs", type=int, default=400,
                     help="jobs of the capacity run (all submitted at once)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default LDE kernel forms instead of timing each first")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="run one device-to-device copy of this size first (HBM counter "
                          "calibration: its FETCH_SIZE/WRITE_SIZE are known bytes)")
@@ -486,6 +507,7 @@ def main() -> int:
         if ev:
             ev[3].record(stream)
 
+    tuned = autotune(args, lib, [(n, w, m)])
     for _ in range(args.warmup):
         step()
     stream.synchronize()
@@ -529,6 +551,7 @@ def main() -> int:
                      "traffic": measured_traffic(args.log_n, w, E),
                      "kernel": "LDE passes (k_step1 + k_lde_step2 + k_lde_step3)",
                      "algorithmic_bytes": lde_bytes},
+        "lde_forms": tuned,
         "tip5": {"permutations": perms,
                  "permutations_per_s": perms / ((hash_ms + merkle_ms) / 1000)},
     }

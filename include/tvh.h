@@ -126,6 +126,33 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t t
                           uint64_t target_offset, size_t target_len, uint64_t* d_out,
                           int use_xfe_randomizers, void* stream);
 
+/* ---- LDE kernel forms and autotuning -------------------------------------- */
+
+/* Kernel variants of the LDE, all with identical results.  A variant code is
+ * form | (tile << 2).  The form is the kernel shape of the fused row step
+ * (inverse row DFT, coset scaling, forward row DFT): TVH_LDE_WIDE runs it as
+ * one kernel with all cosets of a row in one 1024-thread LDS tile;
+ * TVH_LDE_SPLIT runs the inverse row DFTs as a kernel of their own (one more
+ * read and write of the trace-sized data, every thread busy) before the
+ * expansion.  Tile 0/1/2 gives the column steps LDS tiles of 2^14/2^13/2^12
+ * elements (1024/512/256 threads).  The variant is chosen per
+ * (log2 trace_len, log2 expansion); TVH_LDE_AUTO restores the default
+ * (TVH_LDE_WIDE, tile 0).  Batch plans use TVH_LDE_WIDE with tile 0.
+ * No reference counterpart (the reference has one fixed kernel schedule per
+ * size, src/gpu/kernels/ntt_kernel.cu). */
+enum { TVH_LDE_AUTO = 0, TVH_LDE_WIDE = 1, TVH_LDE_SPLIT = 2 };
+int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant);
+int tvh_lde_get_variant(uint32_t log_n, uint32_t log_e, int* variant);
+/* Number of variants tvh_lde_autotune times (the length of its ms array). */
+int tvh_lde_variant_count(void);
+/* Times every variant on scratch buffers of this shape (num_cols capped so
+ * the codewords stay under 4 GiB), keeps the fastest for the shape and returns
+ * its code in *chosen; ms (may be NULL) receives the average milliseconds per
+ * LDE of the variants 1, 2, 5, 6, 9, 10 in that order.  Synchronous; allocates
+ * and frees its own buffers; not to be called concurrently with other work. */
+int tvh_lde_autotune(size_t num_cols, size_t trace_len, size_t target_len, int* chosen,
+                     float* ms);
+
 /* ---- Tip5 (src/gpu/kernels/row_hash_kernel.cu, tip5_kernel.cu) ------------- */
 
 /* digests[r] = Tip5::hash_varlen(row r) over a column-major table.

@@ -64,7 +64,8 @@ def test_commit_batch_matches_single(T, oracle, tvhmod):
         if t["layout"] == 1:
             got, wt = got[5:], wt[5:]  # node 0 is unused in the heap layout
         assert np.array_equal(got, wt), (t["n"], t["w"])
-        assert np.array_equal(T.host(t["cw"]), wcw)
+        if t["w"]:  # a zero-width table has no codewords (its buffer is a 1-element stub)
+            assert np.array_equal(T.host(t["cw"]), wcw)
     for t in tabs:  # small tables against the oracle as well
         if t["n"] * t["w"] <= 4096:
             tr = T.host(t["trace"], t["n"] * t["w"])

@@ -1,0 +1,100 @@
+"""Every LDE kernel variant (row-step form TVH_LDE_WIDE / TVH_LDE_SPLIT times the
+column-step tile 2^14 / 2^13 / 2^12) against the oracle on every shape class (rows
+shorter than, equal to and split from 4096; expansions 1..16; randomizers; trace
+offsets), in single-table calls and inside batch plans, and the autotuner's contract."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import LDE_CASES
+
+pytestmark = pytest.mark.gpu
+
+WIDE, SPLIT = 1, 2
+VARIANTS = (1, 2, 5, 6, 9, 10)  # form | tile code << 2
+EXTRA = [(20, 4, 0, 1), (18, 2, 5, 3), (17, 16, 0, 1), (19, 1, 0, 1)]
+
+
+@pytest.fixture
+def variant(T):
+    """Sets the form for a shape; restores the default afterwards."""
+    touched = []
+
+    def set_(logn, E, v):
+        T.lib.lde_set_variant(logn, E.bit_length() - 1, v)
+        touched.append((logn, E.bit_length() - 1))
+
+    yield set_
+    for ln, le in touched:
+        T.lib.lde_set_variant(ln, le, 0)
+
+
+@pytest.mark.parametrize("form", VARIANTS)
+@pytest.mark.parametrize("logn,E,k,toff", LDE_CASES)
+def test_lde_forms_match_oracle(T, oracle, variant, form, logn, E, k, toff):
+    variant(logn, E, form)
+    n, cols = 1 << logn, 3
+    m = E * n
+    x = oracle.splitmix(5000 + logn, n * cols)
+    r = oracle.splitmix(6000 + k, k * cols) if k else None
+    out = T.empty(m * cols)
+    T.lib.lde_batch(T.dev(x), cols, n, T.dev(r) if k else None, k, toff, 7, m, out)
+    want = oracle.lde_table(x, n, cols, m, toff, 7, None if r is None else r.reshape(cols, k))
+    assert np.array_equal(T.host(out), want)
+
+
+@pytest.mark.parametrize("logn,E,k,toff", EXTRA)
+def test_lde_forms_agree_large(T, oracle, variant, logn, E, k, toff):
+    """Large shapes: the two forms give identical codewords, and sampled codewords
+    match the oracle's direct evaluation."""
+    n, cols = 1 << logn, 2
+    m = E * n
+    x = T.empty(n * cols)
+    T.lib.fill_uniform(x, n * cols, 40 + logn)
+    r = None
+    if k:
+        r = T.empty(k * cols)
+        T.lib.fill_uniform(r, k * cols, 50 + logn)
+    outs = []
+    for form in VARIANTS:
+        variant(logn, E, form)
+        out = T.empty(m * cols)
+        T.lib.lde_batch(x, cols, n, r, k, toff, 7, m, out)
+        outs.append(T.host(out))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+    # sampled check of column 1 against the oracle's coset evaluation of the
+    # interpolant (the full column LDE on the CPU is a few seconds at these sizes)
+    col = T.host(x)[n:2 * n]
+    rr = None if r is None else T.host(r)[k:2 * k]
+    want = oracle.lde_column(col, m, toff, 7, rr)
+    assert np.array_equal(outs[0][m:2 * m], want)
+
+
+def test_batch_plan_with_both_forms(T, oracle, variant):
+    """A batch whose tables use different forms equals per-table commits."""
+    shapes = [(10, 5, 4, WIDE), (12, 3, 4, SPLIT), (13, 7, 2, SPLIT | 4), (8, 11, 8, WIDE | 8),
+              (14, 2, 4, SPLIT | 8), (11, 4, 4, SPLIT), (15, 3, 2, WIDE | 4)]
+    tabs = []
+    for i, (lg, w, E, form) in enumerate(shapes):
+        variant(lg, E, form)
+        n, m = 1 << lg, E << lg
+        tr = T.empty(n * w)
+        T.lib.fill_uniform(tr, n * w, 700 + i)
+        tabs.append(dict(n=n, m=m, w=w, tr=tr, cw=T.empty(m * w), tree=T.empty(5 * (2 * m - 1))))
+    descs = [T.lib.table_desc(t["tr"], t["w"], t["n"], t["m"], t["cw"], t["tree"]) for t in tabs]
+    T.lib.commit_batch(descs)
+    T.sync()
+    for t, (lg, w, E, form) in zip(tabs, shapes):
+        x = T.host(t["tr"])
+        root, cw, _ = oracle.commit(x, t["n"], w, t["m"], want_codewords=True)
+        assert np.array_equal(T.host(t["cw"]), cw), (lg, w, E, form)
+        assert np.array_equal(T.host(t["tree"]).reshape(-1, 5)[-1], root)
+
+
+def test_autotune_contract(T, variant):
+    chosen, ms = T.lib.lde_autotune(8, 1 << 14, 1 << 16)
+    variant(14, 4, chosen)  # restore afterwards
+    assert chosen in VARIANTS
+    assert T.lib.lde_get_variant(14, 2) == chosen
+    assert all(v > 0 for v in ms.values())
+    assert ms[chosen] == min(ms.values())

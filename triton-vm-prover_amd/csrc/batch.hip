@@ -25,7 +25,7 @@ struct PhaseArrays {
 struct tvh_batch_plan {
     int device = -1;
     void* mem = nullptr;
-    PhaseArrays lde[3];
+    PhaseArrays lde[3];  // step 1, step 2, step 3
     PhaseArrays hash;
     std::vector<PhaseArrays> merkle;
     hipGraph_t graph = nullptr;
@@ -88,7 +88,7 @@ int tvh_batch_plan_create(const tvh_table_desc* tables, size_t count, tvh_batch_
         rs.group_stride = t.rand_len;
         LdePlan P;
         int rc = prepare_lde(t.trace, t.num_cols, t.trace_len, rs, t.trace_offset, t.target_offset,
-                             t.target_len, false, t.codewords, &P);
+                             t.target_len, false, t.codewords, &P, true);
         if (rc) return rc;
         if (t.num_cols) {
             if (P.k1) {
@@ -148,10 +148,10 @@ int tvh_batch_plan_launch(tvh_batch_plan* plan, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     auto at = [&](size_t off) { return (const void*)((const uint8_t*)plan->mem + off); };
     int rc = TVH_OK;
-    for (int p = 0; p < 3 && !rc; p++) {
-        const PhaseArrays& ph = plan->lde[p];
+    for (int q = 0; q < 3 && !rc; q++) {
+        const PhaseArrays& ph = plan->lde[q];
         if (ph.nseg)
-            rc = launch_lde_multi(p + 1, at(ph.args_off), (const u32*)at(ph.start_off), ph.nseg,
+            rc = launch_lde_multi(q + 1, at(ph.args_off), (const u32*)at(ph.start_off), ph.nseg,
                                   ph.blocks, ph.lds, st);
     }
     if (!rc && plan->hash.nseg)

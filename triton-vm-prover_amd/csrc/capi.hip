@@ -197,6 +197,73 @@ int tvh_lde_batch_dev(const uint64_t* trace, size_t cols, size_t n, const uint64
     return launch_lde(trace, cols, n, bfe_rand(rand, k), toff, goff, m, false, out, STREAM(s));
 }
 
+int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant) {
+    return set_lde_variant(log_n, log_e, variant);
+}
+
+int tvh_lde_variant_count(void) { return 6; }
+
+int tvh_lde_get_variant(uint32_t log_n, uint32_t log_e, int* variant) {
+    if (!variant) return set_error(TVH_ERR_INVALID_ARG, "null variant pointer");
+    if (log_n > kMaxLogN || log_e > 4) return set_error(TVH_ERR_INVALID_ARG, "shape");
+    *variant = lde_variant(log_n, log_e);
+    return TVH_OK;
+}
+
+int tvh_lde_autotune(size_t cols, size_t n, size_t m, int* chosen, float* ms) {
+    if (!is_pow2(n) || !is_pow2(m) || m < n || m / n > 16)
+        return set_error(TVH_ERR_NOT_POW2, "autotune shape");
+    if (cols == 0) return set_error(TVH_ERR_INVALID_ARG, "autotune needs num_cols > 0");
+    DeviceState* ds = current_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    u32 logN = log2u(n), logE = log2u(m / n);
+    // enough columns to fill the GPU, at most 4 GiB of codewords
+    size_t tc = cols;
+    size_t cap = (4ull << 30) / (8 * m);
+    if (tc > cap) tc = cap > 0 ? cap : 1;
+    u64 *trace = nullptr, *out = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = TVH_OK;
+    if (hipMalloc((void**)&trace, 8 * n * tc) != hipSuccess ||
+        hipMalloc((void**)&out, 8 * m * tc) != hipSuccess)
+        rc = set_error(TVH_ERR_OOM, "autotune buffers");
+    if (!rc && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = set_error(TVH_ERR_DEVICE, "autotune stream/events");
+    if (!rc) rc = launch_fill_uniform(trace, n * tc, 77, st);
+    const int variants[6] = {1, 2, 5, 6, 9, 10};  // form | tile << 2 (tiles 2^14, 2^13, 2^12)
+    float best = 0;
+    int best_v = LDE_WIDE;
+    int prev = lde_variant(logN, logE);
+    for (int i = 0; i < 6 && !rc; i++) {
+        set_lde_variant(logN, logE, variants[i]);
+        RandSpec none;
+        rc = launch_lde(trace, tc, n, none, 1, 7, m, false, out, st);  // warm-up
+        if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = set_error(TVH_ERR_DEVICE, "event");
+        for (int r = 0; r < 3 && !rc; r++) rc = launch_lde(trace, tc, n, none, 1, 7, m, false, out, st);
+        if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+            rc = set_error(TVH_ERR_DEVICE, "autotune timing");
+        float t = 0;
+        if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = set_error(TVH_ERR_DEVICE, "elapsed");
+        t /= 3;
+        if (ms) ms[i] = t;
+        if (!rc && (i == 0 || t < best)) {
+            best = t;
+            best_v = variants[i];
+        }
+    }
+    set_lde_variant(logN, logE, rc ? prev : best_v);
+    if (chosen) *chosen = rc ? prev : best_v;
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (trace) (void)hipFree(trace);
+    if (out) (void)hipFree(out);
+    return rc;
+}
+
 int tvh_lde_column_dev(const uint64_t* trace, size_t n, const uint64_t* rand, size_t k,
                        uint64_t toff, uint64_t goff, size_t m, uint64_t* out, void* s) {
     return tvh_lde_batch_dev(trace, 1, n, rand, k, toff, goff, m, out, s);

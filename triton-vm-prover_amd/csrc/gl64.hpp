@@ -84,19 +84,73 @@ __host__ __device__ __forceinline__ u64 fold128(u64 lo, u64 hi) {
 // a canonical -> canonical
 __host__ __device__ __forceinline__ u64 neg(u64 a) { return a ? P - a : 0; }
 
-__host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi) {
-#ifdef __HIP_DEVICE_COMPILE__
+// 64x64 -> 128 from four independent 32x32 products (v_mad_u64_u32 on the
+// device): the cross sum S = a1*b0 + a0*b1 is kept as 64 bits plus a carry c,
+// then lo = p00 + (S << 32), hi = p11 + (S >> 32) + (c << 32) + carry(lo).
+// No 32-bit half is widened into a fresh register pair (those moves dominated
+// the chained form p00 -> a0*b1 + (p00 >> 32) -> ...).  Host code runs the
+// same function in the native check against unsigned __int128.
+__host__ __device__ __forceinline__ void mul_wide_4mad(u64 a, u64 b, u64& lo, u64& hi) {
     u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
     u64 p00 = (u64)a0 * b0;
-    u64 m1 = (u64)a0 * b1 + (p00 >> 32);   // < 2^64
-    u64 m2 = (u64)a1 * b0 + (u32)m1;       // < 2^64
-    lo = (m2 << 32) | (u32)p00;
-    hi = (u64)a1 * b1 + (m1 >> 32) + (m2 >> 32);
+    u64 q = (u64)a1 * b1;
+    u64 S;
+    bool c = __builtin_add_overflow((u64)a1 * b0, (u64)a0 * b1, &S);
+    u32 k1, k2, k3, k4;
+    u32 l1 = __builtin_addc((u32)(p00 >> 32), (u32)S, 0u, &k1);
+    u32 h0 = __builtin_addc((u32)q, (u32)(S >> 32), k1, &k2);
+    u32 h1 = __builtin_addc((u32)(q >> 32), 0u, k2, &k3);
+    h1 = __builtin_addc(h1, 0u, (u32)c, &k4);
+    lo = mk64((u32)p00, l1);
+    hi = mk64(h0, h1);
+}
+
+__host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi) {
+#ifdef __HIP_DEVICE_COMPILE__
+    mul_wide_4mad(a, b, lo, hi);
 #else
     unsigned __int128 pr = (unsigned __int128)a * b;
     lo = (u64)pr;
     hi = (u64)(pr >> 64);
 #endif
+}
+
+// Montgomery reduction for R = 2^64: x = hi*2^64 + lo -> x * 2^-64 mod p, for
+// any 128-bit x (the reference's montyred, b_field_element.hpp:103).  With
+// m = lo * (1 + 2^32) mod 2^64 (lo + (lo << 32), overflow e), x + m*p is a
+// multiple of 2^64 and (x + m*p) / 2^64 = hi - b with b = m - (m >> 32) - e;
+// a borrow is corrected by adding p (subtracting 2^32 - 1 mod 2^64).  The
+// result is < 2^64 and canonical when hi < p.
+// In 32-bit words: a = (lo0, lo1 + lo0) with e the carry of the high word,
+// b = a - a1 - e, r = hi - b, then r - (2^32 - 1) on a borrow.
+__host__ __device__ __forceinline__ u64 montyred(u64 lo, u64 hi) {
+    u32 lo0 = (u32)lo, lo1 = (u32)(lo >> 32);
+    u32 e, k0, k1, c0, c1, d0, d1;
+    u32 a1 = __builtin_addc(lo1, lo0, 0u, &e);
+    u32 b0 = __builtin_subc(lo0, a1, e, &k0);
+    u32 b1 = __builtin_subc(a1, 0u, k0, &k1);
+    u32 r0 = __builtin_subc((u32)hi, b0, 0u, &c0);
+    u32 r1 = __builtin_subc((u32)(hi >> 32), b1, c0, &c1);
+    r0 = __builtin_subc(r0, 0u - c1, 0u, &d0);
+    r1 = __builtin_subc(r1, 0u, d0, &d1);
+    return mk64(r0, r1);
+}
+
+// x * 2^64 mod p (into Montgomery form), any u64 x -> < 2^64:
+// x*2^64 = x*(2^32 - 1) = x0*2^32 - x0 - x1 (mod p) for x = x1*2^32 + x0.
+__host__ __device__ __forceinline__ u64 to_mont(u64 x) {
+    u32 x0 = (u32)x, x1 = (u32)(x >> 32);
+    u64 t = (u64)x0 * EPS;  // < 2^64
+    u64 r = t - x1;
+    if (t < x1) r -= EPS;   // borrow (only when x0 = 0): add p
+    return r;
+}
+
+// a * b * 2^-64 mod p (Montgomery product), any u64 inputs -> < 2^64
+__host__ __device__ __forceinline__ u64 mont_mul(u64 a, u64 b) {
+    u64 lo, hi;
+    mul_wide(a, b, lo, hi);
+    return montyred(lo, hi);
 }
 
 // a, b any u64 -> lazy
@@ -114,27 +168,6 @@ __host__ __device__ constexpr u64 pow2_mod_p(int e) {
     u64 r = 1;
     for (int i = 0; i < e; i++) r = (r >= P - r) ? r - (P - r) : r + r;
     return r;
-}
-
-// 64x64 -> 128 product from four v_mad_u64_u32 and a carry chain.
-__host__ __device__ __forceinline__ void mul_wide_cc(u64 a, u64 b, u64& lo, u64& hi) {
-    u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
-    u64 p00 = (u64)a0 * b0, p01 = (u64)a0 * b1, p10 = (u64)a1 * b0, p11 = (u64)a1 * b1;
-    u32 k0, K, k2, k3, k4;
-    u32 m0 = __builtin_addc((u32)p01, (u32)p10, 0u, &k0);
-    u32 m1 = __builtin_addc((u32)(p01 >> 32), (u32)(p10 >> 32), k0, &K);   // mid = p01 + p10 (65 bits)
-    u32 l1 = __builtin_addc((u32)(p00 >> 32), m0, 0u, &k2);
-    u32 h0 = __builtin_addc((u32)p11, m1, k2, &k3);
-    u32 h1 = __builtin_addc((u32)(p11 >> 32), K, k3, &k4);
-    lo = mk64((u32)p00, l1);
-    hi = mk64(h0, h1);
-}
-
-// a, b any u64 -> lazy, carry-chain product
-__host__ __device__ __forceinline__ u64 mul_lazy_cc(u64 a, u64 b) {
-    u64 lo, hi;
-    mul_wide_cc(a, b, lo, hi);
-    return fold128(lo, hi);
 }
 
 // x any u64, 0 < E <= 32 -> lazy x * 2^E: lo = x << E, hl = x >> (64 - E) < 2^32
@@ -157,6 +190,40 @@ __host__ __device__ __forceinline__ u64 mul_pow2_small(u64 x) {
 __host__ __device__ __forceinline__ u64 mul_2_32(u64 x) {
     // x = x1*2^32 + x0 ; x*2^32 = x0*2^32 + x1*2^64 = x0*2^32 + x1*(2^32-1)
     return fold128(x << 32, x >> 32);
+}
+
+// x * 2^E mod p for a compile-time 0 <= E < 96 and any u64 x -> canonical,
+// without a general product.  p divides 2^96 + 1, so with x << E split into
+// 32-bit words w0..w4 (E = 32q + r), the 96 low bits count positively and the
+// words above bit 96 negatively: x*2^E = lo + w2*2^64 - hi, lo = (w1:w0),
+// hi = (w4:w3), and 2^64 = 2^32 - 1 mod p turns w2*2^64 into one
+// multiply-add.  The word shuffles are free; r != 0 costs funnel shifts.
+template <int E>
+__host__ __device__ __forceinline__ u64 mul_pow2(u64 x) {
+    static_assert(E >= 0 && E < 96, "exponent");
+    constexpr int q = E / 32, r = E % 32;
+    const u32 x0 = (u32)x, x1 = (u32)(x >> 32);
+    u32 w[5] = {0, 0, 0, 0, 0};
+    if (r == 0) {
+        w[q] = x0;
+        w[q + 1] = x1;
+    } else {
+        w[q] = x0 << r;
+        w[q + 1] = (x1 << r) | (x0 >> (32 - r));
+        w[q + 2] = x1 >> (32 - r);
+    }
+    u64 lo = mk64(w[0], w[1]);
+    u64 hi = mk64(w[3], w[4]);
+    // t = lo + w2 * (2^32 - 1): < 2^64 + 2^64, one carry folded back as 2^64 = EPS
+    u64 m = (u64)w[2] * EPS;
+    u64 t = lo + m;
+    if (t < m) t += EPS;  // cannot carry again: t < 2^64 - 2^33 here
+    if (q > 0) {          // hi = 0 when E < 32
+        u64 u = t - hi;
+        if (t < hi) u -= EPS;  // borrow: add p (u >= 2^63 > EPS before the fix)
+        t = u;
+    }
+    return canon(t);
 }
 
 // x canonical, k runtime in [0, 192) -> canonical x * 2^k (host and tables)

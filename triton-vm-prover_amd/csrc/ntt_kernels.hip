@@ -19,8 +19,10 @@
 // Sub-DFTs of length L <= 4096 run in LDS as Stockham radix-16 passes: each
 // thread holds R values, applies w_(Ns*R)^(k*r) from a table and an R-point
 // DFT whose twiddles are powers of two (w_64 = 2^39), natural order in/out.
-#include <mutex>
+#include <atomic>
 #include <map>
+#include <utility>
+#include <mutex>
 #include <tuple>
 
 #include "tvh_internal.hpp"
@@ -42,57 +44,58 @@ __device__ __forceinline__ constexpr int brev(int i) {
     return r;
 }
 
-// x canonical times the twiddle 2^e, e in [0, 96) known after unrolling ->
-// canonical.  e <= 32 is a shift and one fold; larger e multiply by the
-// constant 2^e mod p.
-__device__ __forceinline__ u64 mul_pow2_c(u64 x, int e) {
-    switch (e) {
-        case 0: return x;
-        case 12: return gl::canon(gl::mul_pow2_small<12>(x));
-        case 24: return gl::canon(gl::mul_pow2_small<24>(x));
-        default: return gl::mul(x, gl::pow2_mod_p(e));
+// R-point DFT in registers, natural order in and out: radix-2 DIT on the
+// bit-reversed input.  The butterflies of stage LEN use w_LEN^j =
+// 2^(39*(64/LEN)*j mod 192) (w_64 = 2^39); a twiddle 2^e with e >= 96 is
+// -2^(e-96), folded into the butterfly by swapping add and sub; 2^e itself is
+// applied by word shifts (gl::mul_pow2).  Every index and exponent is a
+// template constant (index sequences), so the whole DFT is straight-line code.
+template <bool INV, int LEN, int J>
+__device__ __forceinline__ void dft_bfly(u64& x, u64& y) {
+    constexpr int e0 = (39 * (64 / LEN) * J) % 192;
+    constexpr int e = INV ? (192 - e0) % 192 : e0;
+    constexpr bool negw = e >= 96;
+    constexpr int ep = negw ? e - 96 : e;
+    u64 t = ep == 0 ? y : gl::mul_pow2<ep>(y);
+    u64 u = x;
+    if (negw) {
+        x = gl::sub(u, t);
+        y = gl::add(u, t);
+    } else {
+        x = gl::add(u, t);
+        y = gl::sub(u, t);
     }
 }
 
-// R-point DFT in registers, natural order in and out: radix-2 DIT on the
-// bit-reversed input.  w_len^j = 2^(39*(64/len)*j mod 192); a twiddle 2^e with
-// e >= 96 is -2^(e-96), folded into the butterfly by swapping add and sub.
+template <bool INV, int LEN, size_t... K>
+__device__ __forceinline__ void dft_stage(u64* a, std::index_sequence<K...>) {
+    // butterfly K: group K / (LEN/2), offset j = K % (LEN/2)
+    (dft_bfly<INV, LEN, (int)(K % (LEN / 2))>(a[(K / (LEN / 2)) * LEN + K % (LEN / 2)],
+                                             a[(K / (LEN / 2)) * LEN + K % (LEN / 2) + LEN / 2]),
+     ...);
+}
+
+template <int LOGR, bool INV, size_t... S>
+__device__ __forceinline__ void dft_stages(u64* a, std::index_sequence<S...>) {
+    (dft_stage<INV, (2 << S)>(a, std::make_index_sequence<(1 << LOGR) / 2>{}), ...);
+}
+
 template <int LOGR, bool INV>
 __device__ __forceinline__ void dft_reg(u64* v) {
     constexpr int R = 1 << LOGR;
     u64 a[R];
 #pragma unroll
     for (int i = 0; i < R; i++) a[i] = v[brev<LOGR>(i)];
-#pragma unroll
-    for (int len = 2; len <= R; len <<= 1) {
-#pragma unroll
-        for (int i = 0; i < R; i += len) {
-#pragma unroll
-            for (int j = 0; j < len / 2; j++) {
-                int e = (39 * (64 / len) * j) % 192;
-                if (INV) e = (192 - e) % 192;
-                bool negw = e >= 96;
-                u64 u = a[i + j];
-                u64 t = mul_pow2_c(a[i + j + len / 2], negw ? e - 96 : e);
-                if (negw) {
-                    a[i + j] = gl::sub(u, t);
-                    a[i + j + len / 2] = gl::add(u, t);
-                } else {
-                    a[i + j] = gl::add(u, t);
-                    a[i + j + len / 2] = gl::sub(u, t);
-                }
-            }
-        }
-    }
+    dft_stages<LOGR, INV>(a, std::make_index_sequence<LOGR>{});
 #pragma unroll
     for (int i = 0; i < R; i++) v[i] = a[i];
 }
 
-template <int LOGR, bool INV>
+template <int LOGR, bool INV, int NTPB = TPB, int NTILE = TILE>
 __device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT, int Ns,
                                               const u64* __restrict__ r4096) {
     constexpr int R = 1 << LOGR;
-    constexpr int MAXIT = TILE / (R * TPB);
+    constexpr int MAXIT = NTILE / (R * NTPB);
     const int per = 1 << (logL - LOGR);
     const int total = NT << (logL - LOGR);
     const int step = 4096 / (Ns * R);
@@ -100,7 +103,7 @@ __device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT
     u64 v[MAXIT][R];
 #pragma unroll
     for (int it = 0; it < MAXIT; it++) {
-        int w = threadIdx.x + it * TPB;
+        int w = threadIdx.x + it * NTPB;
         if (w < total) {
             int t = w >> (logL - LOGR);
             int j = w & (per - 1);
@@ -117,7 +120,7 @@ __device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < MAXIT; it++) {
-        int w = threadIdx.x + it * TPB;
+        int w = threadIdx.x + it * NTPB;
         if (w < total) {
             int j = (w) & (per - 1);
             int t = w >> (logL - LOGR);
@@ -130,20 +133,21 @@ __device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT
     __syncthreads();
 }
 
-// NT transforms of length 2^logL (<= 4096) in LDS; NT * L <= TILE.
-template <bool INV>
+// NT transforms of length 2^logL (<= 4096) in LDS; NT * L <= NTILE; NTPB
+// threads hold NTILE / NTPB values each.
+template <bool INV, int NTPB = TPB, int NTILE = TILE>
 __device__ __forceinline__ void lds_dft(u64* buf, int logL, int NT, const u64* __restrict__ r4096) {
     const int Lp = padded_len(1 << logL);
     int Ns = 1;
     int rem = logL;
     while (rem >= 4) {
-        stockham_pass<4, INV>(buf, logL, Lp, NT, Ns, r4096);
+        stockham_pass<4, INV, NTPB, NTILE>(buf, logL, Lp, NT, Ns, r4096);
         Ns <<= 4;
         rem -= 4;
     }
-    if (rem == 3) stockham_pass<3, INV>(buf, logL, Lp, NT, Ns, r4096);
-    else if (rem == 2) stockham_pass<2, INV>(buf, logL, Lp, NT, Ns, r4096);
-    else if (rem == 1) stockham_pass<1, INV>(buf, logL, Lp, NT, Ns, r4096);
+    if (rem == 3) stockham_pass<3, INV, NTPB, NTILE>(buf, logL, Lp, NT, Ns, r4096);
+    else if (rem == 2) stockham_pass<2, INV, NTPB, NTILE>(buf, logL, Lp, NT, Ns, r4096);
+    else if (rem == 1) stockham_pass<1, INV, NTPB, NTILE>(buf, logL, Lp, NT, Ns, r4096);
 }
 
 // w_N^(+-x) for x < N via the two-level 2^24 tables.
@@ -167,8 +171,9 @@ __device__ __forceinline__ u32 find_segment(const u32* __restrict__ start, u32 n
 // ---------------------------------------------------------------- step 1
 // Transforms s = (c, n2), element n1 at src[c*sstride + N2*n1 + n2]; DFT over n1
 // (inverse if INV) -> k1; times w_N^(+-n2*k1); to dst[c*dstride + E*N2*k1 + n2].
-template <bool INV>
+template <bool INV, int LT = LOG_TILE>
 __device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Roots& rt) {
+    constexpr int NTPB = 1 << (LT - 4);
     extern __shared__ u64 lds[];
     const u32 logN = A.logN, a = A.a, logNT = A.logNT;
     const u32 b = logN - a;
@@ -176,7 +181,7 @@ __device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Ro
     const int Lp = padded_len(L);
     const int NT = 1 << logNT;
     const u64 s0 = blk << logNT;
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
         int t = e & (NT - 1), i = e >> logNT;
         u64 s = s0 + t;
         if (s < A.ntr) {
@@ -185,8 +190,8 @@ __device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Ro
         }
     }
     __syncthreads();
-    lds_dft<INV>(lds, a, NT, rt.r4096[INV]);
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+    lds_dft<INV, NTPB, 1 << LT>(lds, a, NT, rt.r4096[INV]);
+    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
         int t = e & (NT - 1), k1 = e >> logNT;
         u64 s = s0 + t;
         if (s < A.ntr) {
@@ -198,9 +203,9 @@ __device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Ro
     }
 }
 
-template <bool INV>
-__global__ void __launch_bounds__(TPB) k_step1(Step1Args A, Roots rt) {
-    step1_body<INV>(A, blockIdx.x, rt);
+template <bool INV, int LT = LOG_TILE>
+__global__ void __launch_bounds__(1 << (LT - 4), 4) k_step1(Step1Args A, Roots rt) {
+    step1_body<INV, LT>(A, blockIdx.x, rt);
 }
 
 template <bool INV>
@@ -208,7 +213,7 @@ __global__ void __launch_bounds__(TPB) k_step1_multi(const Step1Args* __restrict
                                                      const u32* __restrict__ start, u32 nseg,
                                                      Roots rt) {
     u32 seg = find_segment(start, nseg, blockIdx.x);
-    Step1Args A = args[seg];
+    const Step1Args& A = args[seg];
     step1_body<INV>(A, blockIdx.x - start[seg], rt);
 }
 
@@ -251,9 +256,18 @@ __device__ __forceinline__ u64 rand_term(const LdeArgs& A, u64 c, u32 s, u64 r) 
     if (!((rs.limb_mask >> limb) & 1)) return 0;
     const u64* p = rs.ptr + (c / rs.group) * rs.group_stride + limb * rs.limb_stride +
                    r * rs.elem_stride;
-    return gl::mul(gl::mul(*p, A.delta_n[s]), gl::pow(A.toff, r));
+    // select delta_n[s] without a dynamic index (a dynamic index into the
+    // by-value kernel argument would copy it to scratch)
+    u64 d = A.delta_n[0];
+#pragma unroll
+    for (u32 i = 1; i < 16; i++)
+        if (s == i) d = A.delta_n[i];
+    return gl::mul(gl::mul(*p, d), gl::pow(A.toff, r));
 }
 
+// INTT = false: the rows already hold the inverse-transformed coefficients
+// (written by k_lde_rowinv), only the coset expansion runs here.
+template <bool INTT = true>
 __device__ __forceinline__ void lde_step2_body(const LdeArgs& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
     u64* out = A.out;
@@ -273,7 +287,7 @@ __device__ __forceinline__ void lde_step2_body(const LdeArgs& A, u64 blk, const 
         }
     }
     __syncthreads();
-    lds_dft<true>(lds, b, NT, rt.r4096[1]);
+    if (INTT) lds_dft<true>(lds, b, NT, rt.r4096[1]);
     // expand: transform (sl*NT + t) <- f_sl for row t
     for (int e = threadIdx.x; e < NT * L; e += TPB) {
         int t = e >> b, k2 = e & (L - 1);
@@ -309,7 +323,43 @@ __device__ __forceinline__ void lde_step2_body(const LdeArgs& A, u64 blk, const 
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_step2(LdeArgs A, Roots rt) {
-    lde_step2_body(A, blockIdx.x, rt);
+    lde_step2_body<true>(A, blockIdx.x, rt);
+}
+
+// Split form of the fused row step: k_lde_rowinv inverse-transforms TILE / N2
+// rows per block with every thread busy and writes the coefficients where the
+// row was (out[c*M + E*N2*k1 + n2]); k_lde_expand then runs the coset
+// expansion of k_lde_step2 on them.  One extra read and write of N elements
+// per column buys full occupancy for the inverse row DFT, which the fused
+// kernel runs with a quarter of its threads at E = 4.
+__global__ void __launch_bounds__(TPB) k_lde_rowinv(LdeArgs A, u32 logNT, Roots rt) {
+    extern __shared__ u64 lds[];
+    const u32 a = A.a, b = A.logN - A.a;
+    const int L = 1 << b;
+    const int Lp = padded_len(L);
+    const int NT = 1 << logNT;
+    const u64 nrows = A.cols << a;
+    const u64 amask = (1ull << a) - 1;
+    const u64 s0 = (u64)blockIdx.x << logNT;
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e >> b, i = e & (L - 1);
+        u64 s = s0 + t;
+        if (s < nrows)
+            lds[pidx(t, i, Lp)] =
+                A.src[(s >> a) * A.src_col_stride + (s & amask) * A.src_row_stride + i];
+    }
+    __syncthreads();
+    lds_dft<true>(lds, b, NT, rt.r4096[1]);
+    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+        int t = e >> b, i = e & (L - 1);
+        u64 s = s0 + t;
+        if (s < nrows)
+            A.out[(s >> a) * A.M + (((u64)A.E * (s & amask)) << b) + i] = lds[pidx(t, i, Lp)];
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_expand(LdeArgs A, Roots rt) {
+    lde_step2_body<false>(A, blockIdx.x, rt);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_step2_multi(const LdeArgs* __restrict__ args,
@@ -317,13 +367,15 @@ __global__ void __launch_bounds__(TPB) k_lde_step2_multi(const LdeArgs* __restri
                                                          Roots rt) {
     u32 seg = find_segment(start, nseg, blockIdx.x);
     const LdeArgs& A = args[seg];  // read in place: a private copy would spill delta_n
-    lde_step2_body(A, blockIdx.x - start[seg], rt);
+    lde_step2_body<true>(A, blockIdx.x - start[seg], rt);
 }
 
 // ---------------------------------------------------------------- LDE step 3
 // Transforms (c, q = u1*E + s), element k1 at out[c*M + E*N2*k1 + q]; forward
 // DFT over k1 -> u2, written back to out[c*M + E*N2*u2 + q].
+template <int LT = LOG_TILE>
 __device__ __forceinline__ void lde_step3_body(const Step3Args& A, u64 blk, const Roots& rt) {
+    constexpr int NTPB = 1 << (LT - 4);
     extern __shared__ u64 lds[];
     u64* out = A.out;
     const u32 b = A.logN - A.a, a = A.a, logNT = A.logNT;
@@ -333,7 +385,7 @@ __device__ __forceinline__ void lde_step3_body(const Step3Args& A, u64 blk, cons
     const u32 logQ = b + A.logE;  // transforms per column = E*N2
     const u64 ntr = A.cols << logQ;
     const u64 s0 = blk << logNT;
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
         int t = e & (NT - 1), i = e >> logNT;
         u64 s = s0 + t;
         if (s < ntr) {
@@ -342,8 +394,8 @@ __device__ __forceinline__ void lde_step3_body(const Step3Args& A, u64 blk, cons
         }
     }
     __syncthreads();
-    lds_dft<false>(lds, a, NT, rt.r4096[0]);
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
+    lds_dft<false, NTPB, 1 << LT>(lds, a, NT, rt.r4096[0]);
+    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
         int t = e & (NT - 1), u2 = e >> logNT;
         u64 s = s0 + t;
         if (s < ntr) {
@@ -353,15 +405,16 @@ __device__ __forceinline__ void lde_step3_body(const Step3Args& A, u64 blk, cons
     }
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_step3(Step3Args A, Roots rt) {
-    lde_step3_body(A, blockIdx.x, rt);
+template <int LT = LOG_TILE>
+__global__ void __launch_bounds__(1 << (LT - 4), 4) k_lde_step3(Step3Args A, Roots rt) {
+    lde_step3_body<LT>(A, blockIdx.x, rt);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_step3_multi(const Step3Args* __restrict__ args,
                                                          const u32* __restrict__ start, u32 nseg,
                                                          Roots rt) {
     u32 seg = find_segment(start, nseg, blockIdx.x);
-    Step3Args A = args[seg];
+    const Step3Args& A = args[seg];
     lde_step3_body(A, blockIdx.x - start[seg], rt);
 }
 
@@ -411,6 +464,26 @@ __global__ void k_xfe_relayout(const u64* __restrict__ src, u64* __restrict__ ds
 
 // ---------------------------------------------------------------- host side
 
+// Tuned K2 form per (log2 N, log2 E); 0 = not set.
+static std::atomic<int> g_variant[kMaxLogN + 1][5];
+
+int lde_variant(u32 logN, u32 logE) {
+    int v = (logN <= kMaxLogN && logE <= 4) ? g_variant[logN][logE].load() : 0;
+    return v ? v : LDE_WIDE;
+}
+
+bool lde_variant_valid(int v) {
+    int form = v & 3, tile = v >> 2;
+    return v == LDE_AUTO || ((form == LDE_WIDE || form == LDE_SPLIT) && tile <= 2);
+}
+
+int set_lde_variant(u32 logN, u32 logE, int v) {
+    if (logN > kMaxLogN || logE > 4 || !lde_variant_valid(v))
+        return set_error(TVH_ERR_INVALID_ARG, "lde variant (%u, %u, %d)", logN, logE, v);
+    g_variant[logN][logE].store(v);
+    return TVH_OK;
+}
+
 static u32 choose_split(u32 logN, u32 E) {
     // N2 (the contiguous row step) as long as one tile allows for all E cosets
     // (E * N2 <= TILE, N2 <= 4096); N1 = N / N2 takes the strided step, whose
@@ -422,11 +495,10 @@ static u32 choose_split(u32 logN, u32 E) {
     return logN - b;
 }
 
-static u32 tile_log_nt(u32 logL, u32 cap_log) {
-    // transforms per block: TILE / max(L, 32), at most 2^cap_log
+static u32 tile_log_nt(u32 logL, u32 lt = LOG_TILE) {
+    // transforms per block of a 2^lt tile: 2^lt / max(L, 32)
     u32 l = logL < 5 ? 5 : logL;
-    u32 t = LOG_TILE - l;
-    return t < cap_log ? t : cap_log;
+    return lt - l;
 }
 
 static size_t lds_bytes(u32 logL, u32 logNT) {
@@ -498,8 +570,11 @@ static int lde_consts(DeviceState* ds, u32 logN, u32 a, u32 E, u64 toff, u64 gof
 }
 
 int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
-                u64 m, bool truncate, u64* out, LdePlan* P) {
-    P->k1 = P->k3 = false;
+                u64 m, bool truncate, u64* out, LdePlan* P, bool batch) {
+    P->k1 = P->k3 = P->split = false;
+    P->k2_variant = lde_variant(log2u(n), log2u(m / (n ? n : 1)));
+    if (batch && (P->k2_variant & 3) == LDE_SPLIT) P->k2_variant = LDE_WIDE;  // one kernel per phase
+    P->col_lt = batch ? LOG_TILE : LOG_TILE - (P->k2_variant >> 2);
     P->s1_blocks = P->s2_blocks = P->s3_blocks = 0;
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
@@ -524,7 +599,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
     // map trace -> Y, so K2 then reads the trace itself.
     P->k1 = a > 0;
     if (P->k1) {
-        u32 logNT = tile_log_nt(a, 30);
+        u32 logNT = tile_log_nt(a, P->col_lt);
         u64 ntr = cols << b;
         P->s1 = Step1Args{trace, n, out, m, logN, a, E, ntr, logNT};
         P->s1_blocks = (u32)((ntr + (1ull << logNT) - 1) >> logNT);
@@ -562,11 +637,23 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
         A.src_row_stride = P->k1 ? ((u64)E << b) : 0;
         P->s2_blocks = (u32)((nrows + (1ull << logNT) - 1) >> logNT);
         P->s2_lds = lds_bytes(b, logNT + logE);
+        P->split = (P->k2_variant & 3) == LDE_SPLIT;
+        if (P->split) {  // inverse rows first (TILE / N2 rows per block), then expansion from out
+            u32 li = tile_log_nt(b);
+            if ((1ull << li) > nrows) li = log2u(nrows);
+            P->inv_logNT = li;
+            P->inv_blocks = (u32)((nrows + (1ull << li) - 1) >> li);
+            P->inv_lds = lds_bytes(b, li);
+            P->s2inv = A;
+            A.src = out;
+            A.src_col_stride = m;
+            A.src_row_stride = (u64)E << b;
+        }
     }
     // K3: forward step 2 for all cosets (identity when N1 = 1)
     P->k3 = a > 0;
     if (P->k3) {
-        u32 logNT = tile_log_nt(a, 30);
+        u32 logNT = tile_log_nt(a, P->col_lt);
         u64 ntr = cols << (b + logE);
         P->s3 = Step3Args{out, logN, a, logE, cols, m, logNT};
         P->s3_blocks = (u32)((ntr + (1ull << logNT) - 1) >> logNT);
@@ -578,19 +665,42 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
 int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff, u64 m,
                bool truncate, u64* out, hipStream_t st) {
     LdePlan P;
-    int rc = prepare_lde(trace, cols, n, rs, toff, goff, m, truncate, out, &P);
+    int rc = prepare_lde(trace, cols, n, rs, toff, goff, m, truncate, out, &P, false);
     if (rc || cols == 0) return rc;
     DeviceState* ds = current_device();
+    const dim3 ctpb(1u << (P.col_lt - 4));
     if (P.k1) {
-        hipLaunchKernelGGL(k_step1<true>, dim3(P.s1_blocks), dim3(TPB), P.s1_lds, st, P.s1,
-                           ds->roots);
+        if (P.col_lt == 12)
+            hipLaunchKernelGGL((k_step1<true, 12>), dim3(P.s1_blocks), ctpb, P.s1_lds, st, P.s1,
+                               ds->roots);
+        else if (P.col_lt == 13)
+            hipLaunchKernelGGL((k_step1<true, 13>), dim3(P.s1_blocks), ctpb, P.s1_lds, st, P.s1,
+                               ds->roots);
+        else
+            hipLaunchKernelGGL((k_step1<true, 14>), dim3(P.s1_blocks), ctpb, P.s1_lds, st, P.s1,
+                               ds->roots);
         if ((rc = check_launch())) return rc;
     }
-    hipLaunchKernelGGL(k_lde_step2, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2, ds->roots);
+    if (P.split) {
+        hipLaunchKernelGGL(k_lde_rowinv, dim3(P.inv_blocks), dim3(TPB), P.inv_lds, st, P.s2inv,
+                           P.inv_logNT, ds->roots);
+        if ((rc = check_launch())) return rc;
+        hipLaunchKernelGGL(k_lde_expand, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2,
+                           ds->roots);
+    } else
+        hipLaunchKernelGGL(k_lde_step2, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2,
+                           ds->roots);
     if ((rc = check_launch())) return rc;
     if (P.k3) {
-        hipLaunchKernelGGL(k_lde_step3, dim3(P.s3_blocks), dim3(TPB), P.s3_lds, st, P.s3,
-                           ds->roots);
+        if (P.col_lt == 12)
+            hipLaunchKernelGGL(k_lde_step3<12>, dim3(P.s3_blocks), ctpb, P.s3_lds, st, P.s3,
+                               ds->roots);
+        else if (P.col_lt == 13)
+            hipLaunchKernelGGL(k_lde_step3<13>, dim3(P.s3_blocks), ctpb, P.s3_lds, st, P.s3,
+                               ds->roots);
+        else
+            hipLaunchKernelGGL(k_lde_step3<14>, dim3(P.s3_blocks), ctpb, P.s3_lds, st, P.s3,
+                               ds->roots);
         if ((rc = check_launch())) return rc;
     }
     return TVH_OK;
@@ -638,7 +748,7 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st)
         u64 g = cols - c0 < group ? cols - c0 : group;
         u64* d = data + c0 * n;
         {
-            u32 logNT = tile_log_nt(a, 30);
+            u32 logNT = tile_log_nt(a);
             u64 ntr = g << b;
             u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
             Step1Args A{d, n, scratch, n, logN, a, 1u, ntr, logNT};
@@ -651,7 +761,7 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st)
             if ((rc = check_launch())) return rc;
         }
         {
-            u32 logNT = tile_log_nt(b, 30);
+            u32 logNT = tile_log_nt(b);
             u64 nrows = g << a;
             if ((1ull << logNT) > nrows) logNT = log2u(nrows);
             u64 blocks = (nrows + (1ull << logNT) - 1) >> logNT;
@@ -672,10 +782,14 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st)
 int ntt_set_attributes() {
     const int bytes = 150 * 1024;
     const void* fns[] = {(const void*)k_step1<true>, (const void*)k_step1<false>,
+                         (const void*)k_step1<true, 13>, (const void*)k_step1<true, 12>,
                          (const void*)k_ntt_step2<true>, (const void*)k_ntt_step2<false>,
-                         (const void*)k_lde_step2, (const void*)k_lde_step3,
+                         (const void*)k_lde_step2, (const void*)k_lde_step3<14>,
+                         (const void*)k_lde_step3<13>, (const void*)k_lde_step3<12>,
                          (const void*)k_step1_multi<true>, (const void*)k_lde_step2_multi,
-                         (const void*)k_lde_step3_multi};
+                         (const void*)k_lde_step3_multi, (const void*)k_lde_rowinv,
+                         (const void*)k_lde_expand,
+};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
             return set_error(TVH_ERR_DEVICE, "LDS attribute");

@@ -4,12 +4,14 @@
 // S-box -> circulant MDS -> round constants; S-box = split-and-lookup on lanes
 // 0..3 and x^7 on lanes 4..15.  The arithmetic is re-derived for VALU:
 //
-//  * split-and-lookup: the reference forms the Montgomery value
-//    montyred(v * R^2) = v * R mod p (R = 2^64 mod p = 2^32 - 1), looks up each
-//    byte and maps back with montyred(0, o) = o * R^-1 = -o * 2^32 (mod p).
-//    Both Montgomery reductions return canonical values, so computing the same
-//    canonical residues with shifts is bit-identical.  The 256-byte table sits
-//    in LDS.
+//  * the state is kept in Montgomery form x_M = x * 2^64 mod p (the form the
+//    reference's split-and-lookup is defined on): the reference forms the
+//    canonical Montgomery value montyred(v * R^2) = v * 2^64 mod p, looks up each
+//    byte and maps back with montyred(0, o) = o * 2^-64, so in Montgomery form
+//    the S-box is the byte lookup of the canonical value alone, and x^7 uses
+//    Montgomery products (montyred is cheaper than the 2^64 = 2^32 - 1 fold).
+//    Callers convert absorbed elements in (to_mont) and digests out
+//    (montyred(y, 0), canonical).  The 256-byte table sits in LDS.
 //  * MDS: the 16x16 circulant has 16-bit entries, so each state word is split
 //    into 32-bit halves and both halves are multiplied in f64 (exact: every
 //    partial sum is below 2^52), using the x^8 - 1 / x^8 + 1 split of the
@@ -32,7 +34,7 @@ constexpr int ROUNDS = 5;
 constexpr int DIGEST = 5;
 
 // Round constants of the Tip5 specification (src/hash/tip5.cpp:48-79).
-__constant__ u64 kRoundConstants[ROUNDS * STATE] = {
+constexpr u64 kRoundConstants[ROUNDS * STATE] = {
     0xbd2a3deb61ab60deull, 0xea7df21ad9547ed2ull, 0x900b3677a1de063full, 0x1b46887e876c8677ull,
     0xd364d977889cfb97ull, 0xdc8dfac843699f02ull, 0x375c405d7190db58ull, 0x27924006d2b0d4b1ull,
     0x78dd1172d483cd38ull, 0x3346c66244882a56ull, 0xb0249b279f498aa5ull, 0x94cd51be79338d4dull,
@@ -55,6 +57,21 @@ __constant__ u64 kRoundConstants[ROUNDS * STATE] = {
     0x8af9d44a5f4fdc6bull, 0xd80219cd97c0d762ull, 0x10c9ceba14148ebbull, 0x539bd4c3f2f24474ull,
 };
 
+struct RoundConstantsM {
+    u64 v[ROUNDS * STATE];
+};
+constexpr RoundConstantsM mont_round_constants() {
+    RoundConstantsM t{};
+    for (int i = 0; i < ROUNDS * STATE; i++)
+        t.v[i] = (u64)(((unsigned __int128)kRoundConstants[i] * gl::EPS) % gl::P);
+    return t;
+}
+// The round constants in Montgomery form (c * 2^64 mod p, 2^64 = 2^32 - 1 mod p).
+__constant__ RoundConstantsM kRcMont = mont_round_constants();
+
+// 1 in Montgomery form (2^64 mod p)
+constexpr u64 kOneM = gl::EPS;
+
 // First column of the circulant MDS matrix: SHA-256("Tip5") as 16 LE u16:
 // 61402 1108 28750 33823 7454 43244 53865 12034 56951 27521 41351 40901 12021
 // 59689 26798 17845 (folded into kMdsP / kMdsQ below).
@@ -67,23 +84,24 @@ __device__ __forceinline__ void load_lut(uint8_t* lut) {
     }
 }
 
-__device__ __forceinline__ u64 split_and_lookup(u64 v, const uint8_t* lut) {
-    u64 vc = gl::canon(v);
-    u64 m = gl::sub(gl::canon(gl::mul_2_32(vc)), vc);  // v * (2^32 - 1): Montgomery form
+// S-box of lanes 0..3 on a Montgomery-form value: bytes of the canonical
+// Montgomery value through the table; the result is the Montgomery form of the
+// reference's output (tip5.cpp split_and_lookup).
+__device__ __forceinline__ u64 split_and_lookup_m(u64 v, const uint8_t* lut) {
+    u64 m = gl::canon(v);
     u32 lo = (u32)m, hi = (u32)(m >> 32);
     u32 olo = (u32)lut[lo & 0xff] | ((u32)lut[(lo >> 8) & 0xff] << 8) |
               ((u32)lut[(lo >> 16) & 0xff] << 16) | ((u32)lut[lo >> 24] << 24);
     u32 ohi = (u32)lut[hi & 0xff] | ((u32)lut[(hi >> 8) & 0xff] << 8) |
               ((u32)lut[(hi >> 16) & 0xff] << 16) | ((u32)lut[hi >> 24] << 24);
-    u64 o = ((u64)ohi << 32) | olo;
-    return gl::neg(gl::canon(gl::mul_2_32(o)));        // o * R^-1 = -o * 2^32
+    return ((u64)ohi << 32) | olo;
 }
 
-__device__ __forceinline__ u64 pow7(u64 x) {
-    u64 x2 = gl::mul_lazy(x, x);
-    u64 x4 = gl::mul_lazy(x2, x2);
-    u64 x3 = gl::mul_lazy(x, x2);
-    return gl::mul_lazy(x3, x4);
+__device__ __forceinline__ u64 pow7_m(u64 x) {
+    u64 x2 = gl::mont_mul(x, x);
+    u64 x4 = gl::mont_mul(x2, x2);
+    u64 x3 = gl::mont_mul(x, x2);
+    return gl::mont_mul(x3, x4);
 }
 
 // integer-valued double in [0, 2^52) -> u64
@@ -148,17 +166,22 @@ __device__ __forceinline__ void mds(u64 s[STATE]) {
     }
 }
 
-__device__ __forceinline__ void permutation(u64 s[STATE], const uint8_t* lut) {
+// The permutation on a Montgomery-form state (any u64 words congruent to x_M).
+__device__ __forceinline__ void permutation_m(u64 s[STATE], const uint8_t* lut) {
 #pragma unroll 1
     for (int r = 0; r < ROUNDS; r++) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) s[i] = split_and_lookup(s[i], lut);
+        for (int i = 0; i < 4; i++) s[i] = split_and_lookup_m(s[i], lut);
 #pragma unroll
-        for (int i = 4; i < STATE; i++) s[i] = pow7(s[i]);
+        for (int i = 4; i < STATE; i++) s[i] = pow7_m(s[i]);
         mds(s);
 #pragma unroll
-        for (int i = 0; i < STATE; i++) s[i] = gl::add_lazy(s[i], kRoundConstants[r * STATE + i]);
+        for (int i = 0; i < STATE; i++) s[i] = gl::add_lazy(s[i], kRcMont.v[r * STATE + i]);
     }
 }
+
+// Montgomery form <-> canonical
+__device__ __forceinline__ u64 to_m(u64 x) { return gl::to_mont(x); }
+__device__ __forceinline__ u64 from_m(u64 y) { return gl::montyred(y, 0); }
 
 }  // namespace tip5

@@ -20,9 +20,10 @@ namespace tvh {
 using gl::u32;
 using gl::u64;
 
+// Sponge states are kept in Montgomery form (tip5_dev.hpp); digests leave canonical.
 __device__ __forceinline__ void store_digest(u64* dst, const u64 s[tip5::STATE]) {
 #pragma unroll
-    for (int i = 0; i < tip5::DIGEST; i++) dst[i] = gl::canon(s[i]);
+    for (int i = 0; i < tip5::DIGEST; i++) dst[i] = tip5::from_m(s[i]);
 }
 
 // Segment lookup for multi-table launches (start[nseg] = total blocks).
@@ -49,17 +50,18 @@ __device__ __forceinline__ void hash_rows_body(const HashArgs& A, u64 blk, const
     u32 used = 0;
     for (; used + tip5::RATE <= cols; used += tip5::RATE) {
 #pragma unroll
-        for (int i = 0; i < tip5::RATE; i++) s[i] = __builtin_nontemporal_load(p + (u64)(used + i) * rows);
-        tip5::permutation(s, lut);
+        for (int i = 0; i < tip5::RATE; i++)
+            s[i] = tip5::to_m(__builtin_nontemporal_load(p + (u64)(used + i) * rows));
+        tip5::permutation_m(s, lut);
     }
     u32 rest = cols - used;
 #pragma unroll
     for (int i = 0; i < tip5::RATE; i++) {
         u32 ui = (u32)i;
-        s[i] = ui < rest ? __builtin_nontemporal_load(p + (u64)(used + ui) * rows)
-                         : (ui == rest ? 1ull : 0ull);
+        s[i] = ui < rest ? tip5::to_m(__builtin_nontemporal_load(p + (u64)(used + ui) * rows))
+                         : (ui == rest ? tip5::kOneM : 0ull);
     }
-    tip5::permutation(s, lut);
+    tip5::permutation_m(s, lut);
     store_digest(A.dig + row * tip5::DIGEST, s);
 }
 
@@ -87,7 +89,8 @@ __global__ void __launch_bounds__(256) k_hash_rows_multi(const HashArgs* __restr
 // blocks are absorbed by overwrite, the remainder is carried.  All rows have
 // the same counts, so lanes never diverge.  State is column-major by word
 // (state[i*rows + row]).  The FINAL chunk absorbs the padding block
-// [rest, 1, 0...] and writes the canonical digest (tip5.cpp:246-282).
+// [rest, 1, 0...] and writes the canonical digest (tip5.cpp:246-282).  The
+// carried state is in Montgomery form, the carried elements are canonical.
 template <bool FINAL>
 __global__ void __launch_bounds__(256) k_absorb_chunk(const u64* __restrict__ chunk, u64 rows,
                                                       u32 ccols, u64* __restrict__ state,
@@ -108,17 +111,17 @@ __global__ void __launch_bounds__(256) k_absorb_chunk(const u64* __restrict__ ch
     u32 used = 0;
     for (; used + tip5::RATE <= total; used += tip5::RATE) {
 #pragma unroll
-        for (int i = 0; i < tip5::RATE; i++) s[i] = elem(used + i);
-        tip5::permutation(s, lut);
+        for (int i = 0; i < tip5::RATE; i++) s[i] = tip5::to_m(elem(used + i));
+        tip5::permutation_m(s, lut);
     }
     const u32 rest = total - used;
     if (FINAL) {
 #pragma unroll
         for (int i = 0; i < tip5::RATE; i++) {
             u32 ui = (u32)i;
-            s[i] = ui < rest ? elem(used + ui) : (ui == rest ? 1ull : 0ull);
+            s[i] = ui < rest ? tip5::to_m(elem(used + ui)) : (ui == rest ? tip5::kOneM : 0ull);
         }
-        tip5::permutation(s, lut);
+        tip5::permutation_m(s, lut);
         store_digest(dig + row * tip5::DIGEST, s);
     } else {
         // carried elements first (they may be read from pend itself: read all, then write)
@@ -142,22 +145,22 @@ __global__ void __launch_bounds__(256) k_permutation(u64* __restrict__ st, u64 n
     if (i >= num) return;
     u64 s[tip5::STATE];
 #pragma unroll
-    for (int k = 0; k < tip5::STATE; k++) s[k] = st[i * tip5::STATE + k];
-    tip5::permutation(s, lut);
+    for (int k = 0; k < tip5::STATE; k++) s[k] = tip5::to_m(st[i * tip5::STATE + k]);
+    tip5::permutation_m(s, lut);
 #pragma unroll
-    for (int k = 0; k < tip5::STATE; k++) st[i * tip5::STATE + k] = gl::canon(s[k]);
+    for (int k = 0; k < tip5::STATE; k++) st[i * tip5::STATE + k] = tip5::from_m(s[k]);
 }
 
 __device__ __forceinline__ void hash_pair(const u64* l, const u64* r, u64 s[tip5::STATE],
                                           const uint8_t* lut) {
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-        s[k] = l[k];
-        s[5 + k] = r[k];
+        s[k] = tip5::to_m(l[k]);
+        s[5 + k] = tip5::to_m(r[k]);
     }
 #pragma unroll
-    for (int k = 10; k < tip5::STATE; k++) s[k] = 1;
-    tip5::permutation(s, lut);
+    for (int k = 10; k < tip5::STATE; k++) s[k] = tip5::kOneM;
+    tip5::permutation_m(s, lut);
 }
 
 // out[i] = hash_pair(left[i], right[i])  (Backend::hash_pairs, backend.hpp:110-115)
@@ -188,7 +191,7 @@ __device__ __forceinline__ void merkle_body(const MerkleArgs& A, u64 blk, const 
         hash_pair(A.src + 10 * parent, A.src + 10 * parent + 5, s, lut);
         u64 d[5];
 #pragma unroll
-        for (int k = 0; k < 5; k++) d[k] = gl::canon(s[k]);
+        for (int k = 0; k < 5; k++) d[k] = tip5::from_m(s[k]);
 #pragma unroll
         for (int k = 0; k < 5; k++) {
             A.dst[0][5 * parent + k] = d[k];
@@ -203,7 +206,7 @@ __device__ __forceinline__ void merkle_body(const MerkleArgs& A, u64 blk, const 
         if (t < width) {
             hash_pair(buf + 10 * t, buf + 10 * t + 5, s, lut);
 #pragma unroll
-            for (int k = 0; k < 5; k++) d[k] = gl::canon(s[k]);
+            for (int k = 0; k < 5; k++) d[k] = tip5::from_m(s[k]);
         }
         __syncthreads();
         if (t < width) {

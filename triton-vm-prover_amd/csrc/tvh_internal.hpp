@@ -108,16 +108,32 @@ struct LdeArgs {
     RandSpec rs;
     u32 logNT;
 };
+// LDE kernel variant: bits 0-1 the form of the fused row step (LDE_WIDE: one
+// kernel, all E cosets of a row in one 1024-thread LDS tile; LDE_SPLIT: the
+// inverse row DFTs as a kernel of their own, then the expansion); bits 2-3 the
+// column kernels' tile, 2^(14 - code) elements with 16 per thread.
+enum LdeVariant { LDE_AUTO = 0, LDE_WIDE = 1, LDE_SPLIT = 2 };
+int lde_variant(u32 logN, u32 logE);            // the tuned choice (never LDE_AUTO)
+bool lde_variant_valid(int v);
+int set_lde_variant(u32 logN, u32 logE, int v);  // v = LDE_AUTO restores the default
+
 struct LdePlan {
     bool k1, k3;
+    int k2_variant;  // full variant code (form | tile << 2)
+    bool split;      // LDE_SPLIT: s2inv (k_lde_rowinv) runs before s2 (k_lde_expand)
+    LdeArgs s2inv;
+    u32 inv_logNT, inv_blocks;
+    size_t inv_lds;
+    u32 col_lt;      // log2 tile of the column kernels (K1, K3)
     Step1Args s1;
     LdeArgs s2;
     Step3Args s3;
     u32 s1_blocks, s2_blocks, s3_blocks;
     size_t s1_lds, s2_lds, s3_lds;
 };
+// batch = true keeps the column kernels at the 2^14 tile of the multi-table kernels.
 int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
-                u64 m, bool truncate, u64* out, LdePlan* plan);
+                u64 m, bool truncate, u64* out, LdePlan* plan, bool batch);
 // phase 1/2/3: k_step1_multi<true> / k_lde_step2_multi / k_lde_step3_multi
 int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg, u32 blocks,
                      size_t lds, hipStream_t st);

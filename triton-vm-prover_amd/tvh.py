@@ -70,6 +70,12 @@ class JobTimes(C.Structure):
 
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+LDE_AUTO, LDE_WIDE, LDE_SPLIT = 0, 1, 2
+LDE_VARIANTS = (1, 2, 5, 6, 9, 10)  # form | column tile code << 2
+
+
+def lde_variant_name(v):
+    return {LDE_WIDE: "wide", LDE_SPLIT: "split"}[v & 3] + f"/t{14 - (v >> 2)}"
 
 _SIGS = {
     "tvh_init": [_i32],
@@ -101,6 +107,10 @@ _SIGS = {
     "tvh_fill_uniform_dev": [_vp, _sz, _u64, _vp],
     "tvh_commit_frugal_workspace_elems": [_sz, _sz, _sz, _sz],
     "tvh_commit_frugal_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _sz, _vp, _vp, _i32, _vp],
+    "tvh_lde_set_variant": [C.c_uint32, C.c_uint32, _i32],
+    "tvh_lde_get_variant": [C.c_uint32, C.c_uint32, _vp],
+    "tvh_lde_autotune": [_sz, _sz, _sz, _vp, _vp],
+    "tvh_lde_variant_count": [],
     "tvh_server_create": [_vp, _vp],
     "tvh_server_submit": [_vp, _vp, _vp],
     "tvh_server_poll": [_vp, _u64, _vp],
@@ -283,6 +293,22 @@ class TVH:
 
     def frugal_workspace_elems(self, cols, n, m, chunk):
         return self.lib.tvh_commit_frugal_workspace_elems(cols, n, m, chunk)
+
+    # -- LDE kernel forms -----------------------------------------------------------
+    def lde_set_variant(self, log_n, log_e, variant):
+        self._call("tvh_lde_set_variant", log_n, log_e, variant)
+
+    def lde_get_variant(self, log_n, log_e):
+        v = C.c_int()
+        self._call("tvh_lde_get_variant", log_n, log_e, C.byref(v))
+        return v.value
+
+    def lde_autotune(self, cols, n, m):
+        """-> (chosen variant code, {variant code: ms per LDE})"""
+        chosen = C.c_int()
+        ms = (C.c_float * self.lib.tvh_lde_variant_count())()
+        self._call("tvh_lde_autotune", cols, n, m, C.byref(chosen), ms)
+        return chosen.value, dict(zip(LDE_VARIANTS, list(ms)))
 
     # -- commit server ------------------------------------------------------------
     def server_create(self, hbm_budget_bytes=0, max_inflight=0, max_queue=0):
