@@ -175,6 +175,11 @@ def run_batch(args, mod, lib) -> dict:
         t_graph += ev[4].elapsed_ms(ev[5])
         lib.batch_destroy(plan)
         del tabs
+    # host-buffer drop-in per table (trace up, codewords and digests down), each once
+    t_host = 0.0
+    if not args.no_host_baseline:
+        for i, (lg, w) in enumerate(shapes):
+            t_host += plain_host_seconds(lib, 1 << lg, w, E << lg, 5000 + i)
     return {
         "metric": "batched LDE+Tip5 Merkle commit cells/s (mixed-height tables)",
         "value": cells / (t_graph / 1000), "unit": "cells/s", "n_gpus": 1,
@@ -186,6 +191,8 @@ def run_batch(args, mod, lib) -> dict:
         "cells_per_s": {"per_table": cells / (t_plain / 1000), "batch_plan": cells / (t_plan / 1000),
                         "batch_graph": cells / (t_graph / 1000)},
         "speedup_vs_per_table": t_plain / t_graph,
+        "host_dropin_per_table": None if args.no_host_baseline else {
+            "s": t_host, "cells_per_s": cells / t_host, "batch_graph_speedup": t_host / (t_graph / 1000)},
         "lde_forms": tuned,
         "root_mismatches": mismatches,
     }
@@ -298,6 +305,21 @@ def plain_job(lib, hip_rt, j, stream, want_rows=False):
     return dt, [int(v) for v in root], rows, paths
 
 
+def plain_host_seconds(lib, n, w, m, seed):
+    """The host-buffer drop-in (tvh_commit_host, the CudaBackend-style path of the
+    CPU prover: trace uploaded, codewords and digests downloaded) for one table;
+    seconds of that call.  Inputs are uniform values from numpy, drawn untimed."""
+    import numpy as np
+
+    tr = random_field(n * w, seed)
+    cw = np.empty(m * w, np.uint64)
+    dg = np.empty(m * 5, np.uint64)
+    root = np.empty(5, np.uint64)
+    t0 = time.perf_counter()
+    lib.commit_host(tr, w, n, m, codewords=cw, digests=dg, root=root)
+    return time.perf_counter() - t0
+
+
 def run_serve(args, mod, lib) -> dict:
     """Served stream (Poisson arrivals, latency) and saturated stream (capacity) through
     the commit server, against the plain per-table path timed per height."""
@@ -323,6 +345,9 @@ def run_serve(args, mod, lib) -> dict:
         plain_s[wj["n"]] = best
     plain_total = sum(plain_s[j["n"]] for j in jobs)
     print(f"serve: plain per-height times {plain_s}", file=sys.stderr, flush=True)
+    host_s = {wj["n"]: plain_host_seconds(lib, wj["n"], wj["w"], wj["m"], 11) for wj in warm}
+    host_total = sum(host_s[j["n"]] for j in jobs)
+    print(f"serve: host drop-in per-height times {host_s}", file=sys.stderr, flush=True)
 
     # bit-exactness sample: one job of every height plus the first 8, compared with the
     # plain path on the same seed (root, opened rows, authentication paths)
@@ -353,6 +378,7 @@ def run_serve(args, mod, lib) -> dict:
     sat_cells = sum(j["m"] * j["w"] for j in sat_jobs)
     sat_span = max(r[3] for r in sres) - min(r[0] for r in sres)
     sat_plain = sum(plain_s[j["n"]] for j in sat_jobs)
+    sat_host = sum(host_s[j["n"]] for j in sat_jobs)
     return {
         "metric": "served LDE+commit jobs: cells/s and p50/p99 job latency",
         "value": cells / span, "unit": "cells/s", "n_gpus": 1, "higher_is_better": True,
@@ -369,10 +395,15 @@ def run_serve(args, mod, lib) -> dict:
         "stream_span_s": span, "offered_cells_per_s": cells / max(1e-9, jobs[-1]["t"]),
         "saturated": {"jobs": len(sat_jobs), "cells_per_s": sat_cells / sat_span,
                       "plain_cells_per_s": sat_cells / sat_plain,
-                      "speedup_vs_plain": sat_plain / sat_span},
+                      "speedup_vs_plain": sat_plain / sat_span,
+                      "host_dropin_cells_per_s": sat_cells / sat_host,
+                      "speedup_vs_host_dropin": sat_host / sat_span},
         "plain_per_table": {"cells_per_s": cells / plain_total, "total_s": plain_total,
                             "s_per_job_by_log_height": {str(k.bit_length() - 1): v
                                                         for k, v in plain_s.items()}},
+        "host_dropin_per_table": {"cells_per_s": cells / host_total, "total_s": host_total,
+                                  "s_per_job_by_log_height": {str(k.bit_length() - 1): v
+                                                              for k, v in host_s.items()}},
         "checked_jobs": len(sample), "mismatches_vs_plain": mism, "lde_forms": tuned,
         "wall_s": t_end - t0,
     }
@@ -380,8 +411,15 @@ def run_serve(args, mod, lib) -> dict:
 
 def autotune(args, lib, shapes):
     """Runs the library's LDE autotuner once per (trace_len, target_len) with the
-    width given; returns {log2 trace_len: {"form": chosen, "ms": {form: ms}}}."""
+    width given; returns {log2 trace_len: {"variant": chosen, "ms": {variant: ms}}}.
+    With --lde-variant the given variant is set for every shape instead."""
     out = {}
+    if args.lde_variant:
+        name = sys.modules["tvh"].lde_variant_name
+        for n, w, m in sorted(set(shapes)):
+            lib.lde_set_variant(n.bit_length() - 1, (m // n).bit_length() - 1, args.lde_variant)
+            out[n.bit_length() - 1] = {"variant": name(args.lde_variant), "forced": True}
+        return out
     if args.no_autotune:
         return out
     for n, w, m in sorted(set(shapes)):
@@ -434,8 +472,13 @@ def main() -> int:
     ap.add_argument("--saturated-jobs", type=int, default=400,
                     help="jobs of the capacity run (all submitted at once)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-host-baseline", action="store_true",
+                    help="batch/serve modes: skip timing the host-buffer drop-in path")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default LDE kernel forms instead of timing each first")
+    ap.add_argument("--lde-variant", type=int, default=0,
+                    help="force this LDE variant code (form | tile << 2) for the workload's shape "
+                         "instead of autotuning (profiling runs)")
     ap.add_argument("--calibrate-copy-gib", type=float, default=0.0,
                     help="run one device-to-device copy of this size first (HBM counter "
                          "calibration: its FETCH_SIZE/WRITE_SIZE are known bytes)")

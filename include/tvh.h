@@ -214,6 +214,30 @@ int tvh_commit_dev(const uint64_t* d_trace, size_t num_cols, size_t trace_len,
                    uint64_t target_offset, size_t target_len, uint64_t* d_codewords,
                    uint64_t* d_tree, int layout, void* stream);
 
+/* ---- FRI codeword commitment and Fiat-Shamir handoff ------------------------ */
+
+/* leaves[i] = (c0, c1, c2, 0, 0) for XFE codeword element i (interleaved
+ * limbs xfe[3i + k]): the FRI round leaves, not hashed.
+ * Replaces qzc_xfe_to_digest_leaves_kernel (gpu_stark.cu:863-875). */
+int tvh_xfe_leaves_dev(const uint64_t* d_xfe, size_t n, uint64_t* d_leaves, void* stream);
+/* Leaves as tvh_xfe_leaves_dev written into the tree's leaf level, then the
+ * Merkle tree of n (power of two) leaves in the given layout (the FRI round
+ * commitment, gpu_stark.cu:5807,5874). */
+int tvh_fri_commit_dev(const uint64_t* d_xfe, size_t n, uint64_t* d_tree, int layout,
+                       void* stream);
+
+/* Device-resident Fiat-Shamir sponge: 16 canonical u64 in device memory.
+ * init = all zeros (variable-length domain, fs_init_sponge_gpu);
+ * absorb = pad [data..., 1, 0...] to a multiple of 10 and, per block of 10,
+ * overwrite the rate and permute (fs_absorb_device_gpu,
+ * fiat_shamir_kernel.cuh:54; Tip5::pad_and_absorb_all); squeeze = out[0..10)
+ * = rate, then permute (fs_squeeze_gpu).  The encoded Merkle root item that
+ * the prover absorbs after a commitment is [0, root0..root4]
+ * (gpu_stark.cu:2576-2587). */
+int tvh_sponge_init_dev(uint64_t* d_state, void* stream);
+int tvh_sponge_absorb_dev(uint64_t* d_state, const uint64_t* d_data, size_t len, void* stream);
+int tvh_sponge_squeeze_dev(uint64_t* d_state, uint64_t* d_out, void* stream);
+
 /* ---- batched commits of many tables ------------------------------------------ */
 
 /* One table commit of a batch: the arguments of tvh_commit_dev. */

@@ -20,6 +20,9 @@ int main() {
     if (gl::mul_lazy(a, b) % P != (uint64_t)(((u128)a * b) % P)) bad++;
     { uint64_t lo, hi; gl::mul_wide_4mad(a, b, lo, hi); u128 w = (u128)a * b;
       if (lo != (uint64_t)w || hi != (uint64_t)(w >> 64)) bad++; }
+    { uint64_t lo, hi; gl::sqr_wide(a, lo, hi); u128 w = (u128)a * a;
+      if (lo != (uint64_t)w || hi != (uint64_t)(w >> 64)) bad++;
+      if (gl::mont_sqr(a) != gl::mont_mul(a, a)) bad++; }
     if (gl::add_lazy(a, bc) % P != (uint64_t)(((u128)a + bc) % P)) bad++;
     {  // Montgomery product of lazy inputs: congruent to a*b*2^-64; canonical if hi < p
       u128 w2 = (u128)a * b;

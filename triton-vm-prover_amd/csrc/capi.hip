@@ -42,13 +42,15 @@ static int build_tables(DeviceState& ds) {
     u64 w4096 = gl::root_of_unity(12), w4096i = gl::inv(w4096);
     u64 w24_4096 = gl::pow(w24, 4096), w24i_4096 = gl::pow(w24i, 4096);
     u64 a = 1, ai = 1, b = 1, bi = 1, c = 1, ci = 1;
+    // stored in Montgomery form (w * 2^64 mod p): the kernels apply them with gl::mont_mul
+    auto m = [](u64 x) { return gl::canon(gl::to_mont(x)); };
     for (int e = 0; e < 4096; e++) {
-        h[0 * 4096 + e] = a;
-        h[1 * 4096 + e] = ai;
-        h[2 * 4096 + e] = b;
-        h[3 * 4096 + e] = bi;
-        h[4 * 4096 + e] = c;
-        h[5 * 4096 + e] = ci;
+        h[0 * 4096 + e] = m(a);
+        h[1 * 4096 + e] = m(ai);
+        h[2 * 4096 + e] = m(b);
+        h[3 * 4096 + e] = m(bi);
+        h[4 * 4096 + e] = m(c);
+        h[5 * 4096 + e] = m(ci);
         a = gl::mul(a, w4096);
         ai = gl::mul(ai, w4096i);
         b = gl::mul(b, w24);
@@ -411,6 +413,42 @@ int tvh_commit_dev(const uint64_t* trace, size_t cols, size_t n, const uint64_t*
     rc = launch_hash_rows(cw, m, cols, leaf_level, STREAM(s));
     if (rc) return rc;
     return launch_merkle_levels(tree, m, layout, STREAM(s));
+}
+
+int tvh_sponge_init_dev(uint64_t* state, void* s) {
+    if (!state) return set_error(TVH_ERR_INVALID_ARG, "null state");
+    if (!current_device()) return TVH_ERR_DEVICE;
+    return launch_sponge(state, nullptr, 0, nullptr, 2, STREAM(s));
+}
+
+int tvh_sponge_absorb_dev(uint64_t* state, const uint64_t* data, size_t len, void* s) {
+    if (!state || (len && !data)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (!current_device()) return TVH_ERR_DEVICE;
+    return launch_sponge(state, data, len, nullptr, 0, STREAM(s));
+}
+
+int tvh_sponge_squeeze_dev(uint64_t* state, uint64_t* out, void* s) {
+    if (!state || !out) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (!current_device()) return TVH_ERR_DEVICE;
+    return launch_sponge(state, nullptr, 0, out, 1, STREAM(s));
+}
+
+int tvh_xfe_leaves_dev(const uint64_t* xfe, size_t n, uint64_t* leaves, void* s) {
+    if (n && (!xfe || !leaves)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (!current_device()) return TVH_ERR_DEVICE;
+    return launch_xfe_leaves(xfe, n, leaves, STREAM(s));
+}
+
+int tvh_fri_commit_dev(const uint64_t* xfe, size_t n, uint64_t* tree, int layout, void* s) {
+    if (layout != TVH_TREE_LEVEL_ORDER && layout != TVH_TREE_HEAP)
+        return set_error(TVH_ERR_LAYOUT, "layout %d", layout);
+    if (!xfe || !tree) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "codeword length must be a power of 2");
+    if (!current_device()) return TVH_ERR_DEVICE;
+    u64* leaf_level = tree + 5 * merkle_level_base(n, 0, layout);
+    int rc = launch_xfe_leaves(xfe, n, leaf_level, STREAM(s));
+    if (rc) return rc;
+    return launch_merkle_levels(tree, n, layout, STREAM(s));
 }
 
 int tvh_fill_uniform_dev(uint64_t* d, size_t count, uint64_t seed, void* s) {

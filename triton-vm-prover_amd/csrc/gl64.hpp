@@ -105,6 +105,25 @@ __host__ __device__ __forceinline__ void mul_wide_4mad(u64 a, u64 b, u64& lo, u6
     hi = mk64(h0, h1);
 }
 
+// a^2 -> 128 bits from three products: the cross term a0*a1 is doubled with
+// word shifts (bit 63 becomes the carry into the high word).
+__host__ __device__ __forceinline__ void sqr_wide(u64 a, u64& lo, u64& hi) {
+    u32 a0 = (u32)a, a1 = (u32)(a >> 32);
+    u64 p00 = (u64)a0 * a0;
+    u64 q = (u64)a1 * a1;
+    u64 x = (u64)a0 * a1;
+    u32 x0 = (u32)x, x1 = (u32)(x >> 32);
+    u32 s0 = x0 << 1, s1 = (x1 << 1) | (x0 >> 31), c = x1 >> 31;  // 2x = c*2^64 + (s1:s0)
+    u32 k1, k2, k3, k4;
+    u32 l1 = __builtin_addc((u32)(p00 >> 32), s0, 0u, &k1);
+    u32 h0 = __builtin_addc((u32)q, s1, k1, &k2);
+    u32 h1 = __builtin_addc((u32)(q >> 32), c, k2, &k3);
+    (void)k3;
+    (void)k4;
+    lo = mk64((u32)p00, l1);
+    hi = mk64(h0, h1);
+}
+
 __host__ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi) {
 #ifdef __HIP_DEVICE_COMPILE__
     mul_wide_4mad(a, b, lo, hi);
@@ -134,6 +153,13 @@ __host__ __device__ __forceinline__ u64 montyred(u64 lo, u64 hi) {
     r0 = __builtin_subc(r0, 0u - c1, 0u, &d0);
     r1 = __builtin_subc(r1, 0u, d0, &d1);
     return mk64(r0, r1);
+}
+
+// a * a * 2^-64 mod p, any u64 a -> < 2^64
+__host__ __device__ __forceinline__ u64 mont_sqr(u64 a) {
+    u64 lo, hi;
+    sqr_wide(a, lo, hi);
+    return montyred(lo, hi);
 }
 
 // x * 2^64 mod p (into Montgomery form), any u64 x -> < 2^64:

@@ -112,7 +112,7 @@ __device__ __forceinline__ void stockham_pass(u64* buf, int logL, int Lp, int NT
             for (int r = 0; r < R; r++) v[it][r] = buf[pidx(t, j + r * per, Lp)];
             if (Ns > 1) {
 #pragma unroll
-                for (int r = 1; r < R; r++) v[it][r] = gl::mul(v[it][r], r4096[k * r * step]);
+                for (int r = 1; r < R; r++) v[it][r] = gl::mont_mul(v[it][r], r4096[k * r * step]);
             }
             dft_reg<LOGR, INV>(v[it]);
         }
@@ -150,10 +150,11 @@ __device__ __forceinline__ void lds_dft(u64* buf, int logL, int NT, const u64* _
     else if (rem == 1) stockham_pass<1, INV, NTPB, NTILE>(buf, logL, Lp, NT, Ns, r4096);
 }
 
-// w_N^(+-x) for x < N via the two-level 2^24 tables.
+// w_N^(+-x) for x < N via the two-level 2^24 tables, in Montgomery form (the
+// tables hold w * 2^64 mod p): apply with gl::mont_mul.
 __device__ __forceinline__ u64 twiddle(const Roots& rt, int dir, u32 x, u32 logN) {
     u32 y = x << (kMaxLogN - logN);
-    return gl::mul(rt.lo[dir][y & 4095], rt.hi[dir][y >> 12]);
+    return gl::mont_mul(rt.lo[dir][y & 4095], rt.hi[dir][y >> 12]);
 }
 
 // Per-block segment lookup for multi-table launches: start[i] is the first
@@ -197,7 +198,7 @@ __device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Ro
         if (s < A.ntr) {
             u64 c = s >> b, n2 = s & ((1ull << b) - 1);
             u64 v = lds[pidx(t, k1, Lp)];
-            if (n2 && k1) v = gl::mul(v, twiddle(rt, INV, (u32)(n2 * k1), logN));
+            if (n2 && k1) v = gl::mont_mul(v, twiddle(rt, INV, (u32)(n2 * k1), logN));
             A.dst[c * A.dstride + (((u64)A.E * k1) << b) + n2] = v;
         }
     }
@@ -299,25 +300,50 @@ __device__ __forceinline__ void lde_step2_body(const LdeArgs& A, u64 blk, const 
             for (u32 sl = 0; sl < E; sl++) {
                 u64 v = C;
                 if (r < A.rs.k) v = gl::add(v, rand_term(A, c, sl, r));
-                u64 sc = gl::mul(A.pa[(sl << a) + k1], A.pb[(sl << b) + k2]);
-                lds[pidx(sl * NT + t, k2, Lp)] = gl::mul(v, sc);
+                lds[pidx(sl * NT + t, k2, Lp)] = gl::mont_mul(v, A.pb[(sl << b) + k2]);
             }
         }
     }
     __syncthreads();
     lds_dft<false>(lds, b, NT * E, rt.r4096[0]);
     // write Z_s[u1][k1] to out[E*(u1 + N2*k1) + s], contiguous over (u1, s)
+    // Output factor pa[s][k1] * w_N^(k1*u1) (N^-1 beta_s^k1 and the four-step
+    // twiddle; pb was applied before the row DFT).
     const int logRW = b + A.logE;  // row of all cosets: E * N2 elements
     const int RW = 1 << logRW;
-    for (int e = threadIdx.x; e < NT * RW; e += TPB) {
-        int t = e >> logRW, q = e & (RW - 1);
-        int u1 = q >> A.logE, sl = q & (E - 1);
-        u64 s = s0 + t;
+    if (RW == TILE) {
+        // one row per block (k1 uniform): thread outputs q = tid + TPB*j have
+        // the fixed coset sl = tid mod E and u1 = tid/E + (TPB/E)*j, so the
+        // factor advances by w_N^(k1*TPB/E) per j
+        u64 s = s0;
         if (s < nrows) {
             u64 c = s >> a, k1 = s & ((1ull << a) - 1);
-            u64 v = lds[pidx(sl * NT + t, u1, Lp)];
-            if (u1 && k1) v = gl::mul(v, twiddle(rt, 0, (u32)(k1 * u1), A.logN));
-            out[c * A.M + (((u64)E * k1) << b) + q] = v;
+            const int sl = threadIdx.x & (E - 1);
+            const u32 u0 = threadIdx.x >> A.logE;
+            u64 f = A.pa[(sl << a) + k1];
+            if (k1 && u0) f = gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * u0), A.logN));
+            const u64 step = twiddle(rt, 0, (u32)(k1 * (TPB >> A.logE)), A.logN);
+            u64* dst = out + c * A.M + (((u64)E * k1) << b);
+#pragma unroll
+            for (int j = 0; j < TILE / TPB; j++) {
+                int q = threadIdx.x + j * TPB;
+                int u1 = q >> A.logE;
+                dst[q] = gl::mont_mul(lds[pidx(sl, u1, Lp)], f);
+                if (j + 1 < TILE / TPB) f = gl::mont_mul(f, step);
+            }
+        }
+    } else {
+        for (int e = threadIdx.x; e < NT * RW; e += TPB) {
+            int t = e >> logRW, q = e & (RW - 1);
+            int u1 = q >> A.logE, sl = q & (E - 1);
+            u64 s = s0 + t;
+            if (s < nrows) {
+                u64 c = s >> a, k1 = s & ((1ull << a) - 1);
+                u64 f = A.pa[(sl << a) + k1];
+                if (u1 && k1) f = gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * u1), A.logN));
+                out[c * A.M + (((u64)E * k1) << b) + q] =
+                    gl::mont_mul(lds[pidx(sl * NT + t, u1, Lp)], f);
+            }
         }
     }
 }
@@ -422,13 +448,14 @@ __global__ void __launch_bounds__(TPB) k_lde_step3_multi(const Step3Args* __rest
 __global__ void k_lde_consts(u64* pa, u64* pb, u32 a, u32 b, u32 E, const u64* beta, u64 ninv) {
     u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     u64 n1 = 1ull << a, n2 = 1ull << b;
+    // stored in Montgomery form (x * 2^64 mod p) for gl::mont_mul
     if (i < E * n1) {
         u64 s = i >> a, k1 = i & (n1 - 1);
-        pa[i] = gl::mul(ninv, gl::pow(beta[s], k1));
+        pa[i] = gl::canon(gl::to_mont(gl::mul(ninv, gl::pow(beta[s], k1))));
     } else if (i < E * (n1 + n2)) {
         u64 j = i - E * n1;
         u64 s = j >> b, k2 = j & (n2 - 1);
-        pb[j] = gl::pow(gl::pow(beta[s], n1), k2);
+        pb[j] = gl::canon(gl::to_mont(gl::pow(gl::pow(beta[s], n1), k2)));
     }
 }
 

@@ -216,9 +216,10 @@ struct tvh_server {
     }
 
     // Gathers tree nodes (heap indices) and, when rows_out is set, codeword rows.
+    // The caller holds qmu (taken before the job lookup, so a concurrent release
+    // of the job waits until the gather has finished).
     int gather(const u64* tree, const u64* cw, size_t m, size_t w, const std::vector<u64>& heap,
                const size_t* rows, size_t nrows, u64* nodes_out, u64* rows_out) {
-        std::lock_guard<std::mutex> g(qmu);
         size_t nh = nodes_out ? heap.size() : 0, nr = rows_out ? nrows : 0;
         if (!nh && !(nr && w)) return TVH_OK;
         // buffer: heap indices | row indices | nodes (5 each) | rows (w each)
@@ -345,6 +346,7 @@ int tvh_server_wait(tvh_server* s, uint64_t id, uint64_t* root, tvh_job_times* t
 int tvh_server_open(tvh_server* s, uint64_t id, const size_t* leaves, size_t count,
                     uint64_t* rows_out, uint64_t* auth_out, size_t auth_cap, size_t* auth_count) {
     if (!s || (count && !leaves)) return set_error(TVH_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> q(s->qmu);  // lock order: qmu, then mu
     const u64 *tree, *cw;
     size_t m, w;
     {
@@ -374,6 +376,7 @@ int tvh_server_open(tvh_server* s, uint64_t id, const size_t* leaves, size_t cou
 int tvh_server_paths(tvh_server* s, uint64_t id, const size_t* leaves, size_t count,
                      uint64_t* paths) {
     if (!s || (count && (!leaves || !paths))) return set_error(TVH_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> q(s->qmu);  // lock order: qmu, then mu
     const u64 *tree, *cw;
     size_t m, w;
     {

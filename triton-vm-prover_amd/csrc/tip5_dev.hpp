@@ -98,8 +98,8 @@ __device__ __forceinline__ u64 split_and_lookup_m(u64 v, const uint8_t* lut) {
 }
 
 __device__ __forceinline__ u64 pow7_m(u64 x) {
-    u64 x2 = gl::mont_mul(x, x);
-    u64 x4 = gl::mont_mul(x2, x2);
+    u64 x2 = gl::mont_sqr(x);
+    u64 x4 = gl::mont_sqr(x2);
     u64 x3 = gl::mont_mul(x, x2);
     return gl::mont_mul(x3, x4);
 }

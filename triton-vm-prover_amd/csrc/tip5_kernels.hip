@@ -284,6 +284,73 @@ int launch_absorb_chunk(const u64* chunk, u64 rows, u64 ccols, u64* state, u64* 
     return check_launch();
 }
 
+// Fiat-Shamir sponge on a device-resident state (16 canonical u64), following
+// the reference's device transcript (fiat_shamir_kernel.cu: fs_absorb_device_gpu
+// pads [data..., 1, 0...] to a multiple of 10 and overwrites the rate before
+// each permutation; squeeze reads the rate, then permutes).  One lane runs the
+// whole absorption in one launch; the other 63 lanes of the wave only help
+// load the S-box table.
+// mode 0 absorb, 1 squeeze, 2 init (zeros; a kernel rather than a memset so
+// that the call can be captured in a graph).
+__global__ void __launch_bounds__(64) k_sponge(u64* __restrict__ state,
+                                               const u64* __restrict__ data, u64 len,
+                                               u64* __restrict__ out, int mode) {
+    if (mode == 2) {
+        if (threadIdx.x < tip5::STATE) state[threadIdx.x] = 0;
+        return;
+    }
+    const bool squeeze = mode == 1;
+    __shared__ uint8_t lut[256];
+    tip5::load_lut(lut);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    u64 s[tip5::STATE];
+#pragma unroll
+    for (int i = 0; i < tip5::STATE; i++) s[i] = tip5::to_m(state[i]);
+    if (squeeze) {
+#pragma unroll
+        for (int i = 0; i < tip5::RATE; i++) out[i] = tip5::from_m(s[i]);
+        tip5::permutation_m(s, lut);
+    } else {
+        u64 padded = (len + 1 + tip5::RATE - 1) / tip5::RATE * tip5::RATE;
+        for (u64 base = 0; base < padded; base += tip5::RATE) {
+#pragma unroll
+            for (int i = 0; i < tip5::RATE; i++) {
+                u64 pos = base + i;
+                s[i] = pos < len ? tip5::to_m(data[pos]) : (pos == len ? tip5::kOneM : 0ull);
+            }
+            tip5::permutation_m(s, lut);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < tip5::STATE; i++) state[i] = tip5::from_m(s[i]);
+}
+
+int launch_sponge(u64* state, const u64* data, u64 len, u64* out, int mode, hipStream_t st) {
+    hipLaunchKernelGGL(k_sponge, dim3(1), dim3(64), 0, st, state, data, len, out, mode);
+    return check_launch();
+}
+
+// FRI codeword leaves (reference gpu_stark.cu:863-875): XFE i (3 limbs,
+// interleaved) becomes the digest (c0, c1, c2, 0, 0) without hashing.
+__global__ void k_xfe_leaves(const u64* __restrict__ xfe, u64 n, u64* __restrict__ leaves) {
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 a = xfe[3 * i], b = xfe[3 * i + 1], c = xfe[3 * i + 2];
+    leaves[5 * i + 0] = a;
+    leaves[5 * i + 1] = b;
+    leaves[5 * i + 2] = c;
+    leaves[5 * i + 3] = 0;
+    leaves[5 * i + 4] = 0;
+}
+
+int launch_xfe_leaves(const u64* xfe, u64 n, u64* leaves, hipStream_t st) {
+    if (n == 0) return TVH_OK;
+    hipLaunchKernelGGL(k_xfe_leaves, dim3((u32)((n + 255) / 256)), dim3(256), 0, st, xfe, n,
+                       leaves);
+    return check_launch();
+}
+
 int launch_permutation(u64* states, u64 num, hipStream_t st) {
     if (num == 0) return TVH_OK;
     hipLaunchKernelGGL(k_permutation, dim3((u32)((num + 255) / 256)), dim3(256), 0, st, states,

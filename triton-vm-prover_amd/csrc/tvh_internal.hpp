@@ -21,6 +21,9 @@ using gl::u64;
 //   r4096[d][e] = w_4096^(+-e)               e < 4096
 //   lo[d][e]    = w^(+-e), hi[d][e] = w^(+-4096 e)   e < 4096
 // so w_N^x = lo[x & 4095] * hi[x >> 12] for x < 2^24 after scaling x by 2^24/N.
+// All entries are stored in Montgomery form (w * 2^64 mod p, canonical) and
+// applied with gl::mont_mul, which returns the canonical product for a
+// canonical operand.
 struct Roots {
     const u64* r4096[2];
     const u64* lo[2];
@@ -101,8 +104,8 @@ struct LdeArgs {
     u64 cols, M;
     const u64* src;    // rows of step 2: src[c*src_col_stride + k1*src_row_stride + n2]
     u64 src_col_stride, src_row_stride;
-    const u64* pa;     // pa[s*N1 + k1] = N^-1 * beta_s^k1
-    const u64* pb;     // pb[s*N2 + k2] = beta_s^(N1*k2)
+    const u64* pa;     // pa[s*N1 + k1] = N^-1 * beta_s^k1      (Montgomery form)
+    const u64* pb;     // pb[s*N2 + k2] = beta_s^(N1*k2)         (Montgomery form)
     u64 delta_n[16];   // (g^N w_E^s - o^N) * N  (or -o^N * N when truncating with E = 1)
     u64 toff;          // o
     RandSpec rs;
@@ -165,6 +168,9 @@ int launch_merkle_multi(const MerkleArgs* d_args, const u32* d_start, u32 nseg, 
                         hipStream_t st);
 int launch_hash_rows(const u64* tab, u64 rows, u64 cols, u64* dig, hipStream_t st);
 int launch_permutation(u64* states, u64 num, hipStream_t st);
+// mode 0 absorb (pad and absorb all), 1 squeeze, 2 init
+int launch_sponge(u64* state, const u64* data, u64 len, u64* out, int mode, hipStream_t st);
+int launch_xfe_leaves(const u64* xfe, u64 n, u64* leaves, hipStream_t st);
 int launch_absorb_chunk(const u64* chunk, u64 rows, u64 ccols, u64* state, u64* pend, u32 npend,
                         bool first, bool final_chunk, u64* dig, hipStream_t st);
 int launch_hash_pairs(const u64* l, const u64* r, u64* out, u64 count, hipStream_t st);

@@ -99,6 +99,11 @@ _SIGS = {
     "tvh_auth_path_indices": [_sz, _sz, _vp, _sz, _vp],
     "tvh_auth_structure_indices": [_sz, _vp, _sz, _vp, _sz, _vp],
     "tvh_commit_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp, _vp, _i32, _vp],
+    "tvh_xfe_leaves_dev": [_vp, _sz, _vp, _vp],
+    "tvh_fri_commit_dev": [_vp, _sz, _vp, _i32, _vp],
+    "tvh_sponge_init_dev": [_vp, _vp],
+    "tvh_sponge_absorb_dev": [_vp, _vp, _sz, _vp],
+    "tvh_sponge_squeeze_dev": [_vp, _vp, _vp],
     "tvh_batch_plan_create": [_vp, _sz, _vp],
     "tvh_batch_plan_launch": [_vp, _vp],
     "tvh_batch_plan_launch_graph": [_vp, _vp],
@@ -255,6 +260,23 @@ class TVH:
                layout=TREE_LEVEL_ORDER, stream=None):
         self._call("tvh_commit_dev", _ptr(trace), cols, n, _ptr(rand), k, toff, goff, m,
                    _ptr(codewords), _ptr(tree), layout, self.stream_ptr(stream))
+
+    # -- FRI leaves and the Fiat-Shamir sponge -------------------------------------
+    def xfe_leaves(self, xfe, n, leaves, stream=None):
+        self._call("tvh_xfe_leaves_dev", _ptr(xfe), n, _ptr(leaves), self.stream_ptr(stream))
+
+    def fri_commit(self, xfe, n, tree, layout=TREE_LEVEL_ORDER, stream=None):
+        self._call("tvh_fri_commit_dev", _ptr(xfe), n, _ptr(tree), layout, self.stream_ptr(stream))
+
+    def sponge_init(self, state, stream=None):
+        self._call("tvh_sponge_init_dev", _ptr(state), self.stream_ptr(stream))
+
+    def sponge_absorb(self, state, data, length, stream=None):
+        self._call("tvh_sponge_absorb_dev", _ptr(state), _ptr(data), length,
+                   self.stream_ptr(stream))
+
+    def sponge_squeeze(self, state, out, stream=None):
+        self._call("tvh_sponge_squeeze_dev", _ptr(state), _ptr(out), self.stream_ptr(stream))
 
     # -- batches ----------------------------------------------------------------
     @staticmethod
