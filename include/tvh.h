@@ -137,7 +137,8 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t t
  * expansion.  Tile 0/1/2 gives the column steps LDS tiles of 2^14/2^13/2^12
  * elements (1024/512/256 threads).  The variant is chosen per
  * (log2 trace_len, log2 expansion); TVH_LDE_AUTO restores the default
- * (TVH_LDE_WIDE, tile 0).  Batch plans use TVH_LDE_WIDE with tile 0.
+ * (TVH_LDE_SPLIT with tile 2 from 2^14 rows on, TVH_LDE_WIDE with tile 2
+ * below).  Batch plans use each table's form with tile 2.
  * No reference counterpart (the reference has one fixed kernel schedule per
  * size, src/gpu/kernels/ntt_kernel.cu). */
 enum { TVH_LDE_AUTO = 0, TVH_LDE_WIDE = 1, TVH_LDE_SPLIT = 2 };

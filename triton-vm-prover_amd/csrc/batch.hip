@@ -25,7 +25,7 @@ struct PhaseArrays {
 struct tvh_batch_plan {
     int device = -1;
     void* mem = nullptr;
-    PhaseArrays lde[3];  // step 1, step 2, step 3
+    PhaseArrays lde[LDE_PHASES];  // indexed by LdePhase, launched in that order
     PhaseArrays hash;
     std::vector<PhaseArrays> merkle;
     hipGraph_t graph = nullptr;
@@ -66,10 +66,10 @@ int tvh_batch_plan_create(const tvh_table_desc* tables, size_t count, tvh_batch_
     DeviceState* ds = current_device();
     if (!ds) return TVH_ERR_DEVICE;
     std::vector<Step1Args> s1;
-    std::vector<LdeArgs> s2;
+    std::vector<LdeArgs> s2w, s2i, s2e;  // wide row step; split form: inverse rows, expansion
     std::vector<Step3Args> s3;
-    std::vector<u32> b1, b2, b3;
-    std::vector<size_t> l1, l2, l3;
+    std::vector<u32> b1, b2w, b2i, b2e, b3;
+    std::vector<size_t> l1, l2w, l2i, l2e, l3;
     std::vector<HashArgs> h;
     std::vector<u32> hb;
     std::vector<std::vector<MerkleArgs>> ma;
@@ -96,9 +96,18 @@ int tvh_batch_plan_create(const tvh_table_desc* tables, size_t count, tvh_batch_
                 b1.push_back(P.s1_blocks);
                 l1.push_back(P.s1_lds);
             }
-            s2.push_back(P.s2);
-            b2.push_back(P.s2_blocks);
-            l2.push_back(P.s2_lds);
+            if (P.split) {
+                s2i.push_back(P.s2inv);
+                b2i.push_back(P.inv_blocks);
+                l2i.push_back(P.inv_lds);
+                s2e.push_back(P.s2);
+                b2e.push_back(P.s2_blocks);
+                l2e.push_back(P.s2_lds);
+            } else {
+                s2w.push_back(P.s2);
+                b2w.push_back(P.s2_blocks);
+                l2w.push_back(P.s2_lds);
+            }
             if (P.k3) {
                 s3.push_back(P.s3);
                 b3.push_back(P.s3_blocks);
@@ -123,9 +132,11 @@ int tvh_batch_plan_create(const tvh_table_desc* tables, size_t count, tvh_batch_
     tvh_batch_plan* plan = new tvh_batch_plan();
     plan->device = ds->device;
     std::vector<uint8_t> blob;
-    add_phase(blob, plan->lde[0], s1, b1, l1);
-    add_phase(blob, plan->lde[1], s2, b2, l2);
-    add_phase(blob, plan->lde[2], s3, b3, l3);
+    add_phase(blob, plan->lde[LDE_PHASE_STEP1], s1, b1, l1);
+    add_phase(blob, plan->lde[LDE_PHASE_ROWINV], s2i, b2i, l2i);
+    add_phase(blob, plan->lde[LDE_PHASE_STEP2], s2w, b2w, l2w);
+    add_phase(blob, plan->lde[LDE_PHASE_EXPAND], s2e, b2e, l2e);
+    add_phase(blob, plan->lde[LDE_PHASE_STEP3], s3, b3, l3);
     add_phase(blob, plan->hash, h, hb, {});
     plan->merkle.resize(ma.size());
     for (size_t g = 0; g < ma.size(); g++) add_phase(blob, plan->merkle[g], ma[g], mb[g], {});
@@ -148,10 +159,10 @@ int tvh_batch_plan_launch(tvh_batch_plan* plan, void* stream) {
     hipStream_t st = (hipStream_t)stream;
     auto at = [&](size_t off) { return (const void*)((const uint8_t*)plan->mem + off); };
     int rc = TVH_OK;
-    for (int q = 0; q < 3 && !rc; q++) {
+    for (int q = 0; q < LDE_PHASES && !rc; q++) {
         const PhaseArrays& ph = plan->lde[q];
         if (ph.nseg)
-            rc = launch_lde_multi(q + 1, at(ph.args_off), (const u32*)at(ph.start_off), ph.nseg,
+            rc = launch_lde_multi(q, at(ph.args_off), (const u32*)at(ph.start_off), ph.nseg,
                                   ph.blocks, ph.lds, st);
     }
     if (!rc && plan->hash.nseg)

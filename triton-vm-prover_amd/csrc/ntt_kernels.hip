@@ -150,6 +150,123 @@ __device__ __forceinline__ void lds_dft(u64* buf, int logL, int NT, const u64* _
     else if (rem == 1) stockham_pass<1, INV, NTPB, NTILE>(buf, logL, Lp, NT, Ns, r4096);
 }
 
+// ---------------------------------------------------------------- fused-endpoint DFT
+// The same Stockham passes as lds_dft, but the first pass takes its inputs
+// straight from global memory (a Load functor) and the last pass writes its
+// outputs straight to global memory (a Store functor): per transform one LDS
+// write and read per middle pass instead of a global->LDS copy before and an
+// LDS->global copy after, and 2 barriers per middle pass only.  A one-pass
+// transform (L <= 16) never touches LDS.
+//
+// Work item (t, j) of a radix-R pass: v[r] = x_t[j + r*per] (times the
+// Stockham twiddle when Ns > 1), R-point DFT, y_t[(j - k)*R + k + r*Ns] = v[r]
+// with k = j mod Ns.  Functors: load(t, j, per, v) fills v[0..R); store(t, j,
+// Ns, v) writes it.  TF (transform-fastest): adjacent threads take adjacent
+// transforms (column kernels, whose transforms are adjacent in memory);
+// otherwise adjacent elements of one transform (row kernels).
+struct LdsIO {
+    u64* buf;
+    int Lp;
+    template <int R>
+    __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = buf[pidx(t, j + r * per, Lp)];
+    }
+    template <int R>
+    __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
+        int k = j & (Ns - 1);
+        int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) buf[pidx(t, base + r * Ns, Lp)] = v[r];
+    }
+};
+
+template <int LOGR, bool INV, int NTPB, int NTILE, bool TF, bool FROM_LDS, bool TO_LDS, class Ld,
+          class St>
+__device__ __forceinline__ void fused_pass(const LdsIO& io, int logL, int NT, int logNT, int Ns,
+                                           const u64* __restrict__ r4096, const Ld& ld,
+                                           const St& st) {
+    constexpr int R = 1 << LOGR;
+    constexpr int MAXIT = NTILE / (R * NTPB);
+    static_assert(MAXIT >= 1, "radix larger than the per-thread element count");
+    const int lp = logL - LOGR;
+    const int per = 1 << lp;
+    const int total = NT << lp;
+    const int step = 4096 / (Ns * R);
+    u64 v[MAXIT][R];
+#pragma unroll
+    for (int it = 0; it < MAXIT; it++) {
+        int w = threadIdx.x + it * NTPB;
+        if (w < total) {
+            int t = TF ? (w & (NT - 1)) : (w >> lp);
+            int j = TF ? (w >> logNT) : (w & (per - 1));
+            if (FROM_LDS) io.load<R>(t, j, per, v[it]);
+            else ld.template load<R>(t, j, per, v[it]);
+            if (Ns > 1) {
+                int k = j & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < R; r++) v[it][r] = gl::mont_mul(v[it][r], r4096[k * r * step]);
+            }
+            dft_reg<LOGR, INV>(v[it]);
+        }
+    }
+    if (FROM_LDS && TO_LDS) __syncthreads();  // in place: every read of the pass before any write
+#pragma unroll
+    for (int it = 0; it < MAXIT; it++) {
+        int w = threadIdx.x + it * NTPB;
+        if (w < total) {
+            int t = TF ? (w & (NT - 1)) : (w >> lp);
+            int j = TF ? (w >> logNT) : (w & (per - 1));
+            if (TO_LDS) io.store<R>(t, j, Ns, v[it]);
+            else st.template store<R>(t, j, Ns, v[it]);
+        }
+    }
+}
+
+// Straight-line pass sequence for L = 2^(4Q + REM): Q radix-16 passes, then a
+// radix-2^REM pass (a radix-1 copy when L = 1).  Every radix, stride and
+// first/last role is a compile-time constant, which keeps register pressure
+// at one pass's worth.
+template <int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, class Ld, class St, size_t... P>
+__device__ __forceinline__ void fused_passes(const LdsIO& io, int NT, int logNT,
+                                             const u64* __restrict__ r4096, const Ld& ld,
+                                             const St& st, std::index_sequence<P...>) {
+    constexpr int NP = (int)sizeof...(P);
+    constexpr int logL = 4 * Q + REM;
+    ((fused_pass<((int)P < Q ? 4 : REM), INV, NTPB, NTILE, TF, (P > 0), ((int)P + 1 < NP)>(
+          io, logL, NT, logNT, 1 << (4 * P), r4096, ld, st),
+      ((int)P + 1 < NP ? __syncthreads() : (void)0)),
+     ...);
+}
+
+// NT (power of two) transforms of length 2^logL (<= 4096), NT * L <= NTILE.
+// The block's LDS (buf) must not be in use by other threads on entry.
+template <bool INV, int NTPB, int NTILE, bool TF, class Ld, class St>
+__device__ __forceinline__ void fused_dft(u64* buf, int logL, int NT, int logNT,
+                                          const u64* __restrict__ r4096, const Ld& ld,
+                                          const St& st) {
+    const LdsIO io{buf, padded_len(1 << logL)};
+#define TVH_FUSED(q, r)                                                                      \
+    fused_passes<q, r, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st,                 \
+                                             std::make_index_sequence<(q) + ((r) != 0 || (q) == 0)>{})
+    switch (logL) {
+        case 0: TVH_FUSED(0, 0); break;
+        case 1: TVH_FUSED(0, 1); break;
+        case 2: TVH_FUSED(0, 2); break;
+        case 3: TVH_FUSED(0, 3); break;
+        case 4: TVH_FUSED(1, 0); break;
+        case 5: TVH_FUSED(1, 1); break;
+        case 6: TVH_FUSED(1, 2); break;
+        case 7: TVH_FUSED(1, 3); break;
+        case 8: TVH_FUSED(2, 0); break;
+        case 9: TVH_FUSED(2, 1); break;
+        case 10: TVH_FUSED(2, 2); break;
+        case 11: TVH_FUSED(2, 3); break;
+        default: TVH_FUSED(3, 0); break;  // 12
+    }
+#undef TVH_FUSED
+}
+
 // w_N^(+-x) for x < N via the two-level 2^24 tables, in Montgomery form (the
 // tables hold w * 2^64 mod p): apply with gl::mont_mul.
 __device__ __forceinline__ u64 twiddle(const Roots& rt, int dir, u32 x, u32 logN) {
@@ -169,39 +286,61 @@ __device__ __forceinline__ u32 find_segment(const u32* __restrict__ start, u32 n
     return lo;
 }
 
+// Transforms s = (c, n2): element i at src[c*sstride + (i << b) + n2].
+struct Step1Load {
+    const u64* src;
+    u64 sstride, s0, ntr;
+    u32 b;
+    template <int R>
+    __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s < ntr) {
+            const u64* p = src + (s >> b) * sstride + (s & ((1ull << b) - 1));
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = p[(u64)(j + r * per) << b];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = 0;
+        }
+    }
+};
+// Output k1 of transform (c, n2), times w_N^(+-n2*k1), to dst[c*dstride + E*N2*k1 + n2].
+template <bool INV>
+struct Step1Store {
+    u64* dst;
+    u64 dstride, s0, ntr;
+    u32 b, E, logN;
+    Roots rt;
+    template <int R>
+    __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s >= ntr) return;
+        u64 n2 = s & ((1ull << b) - 1);
+        u64* p = dst + (s >> b) * dstride + n2;
+        int k = j & (Ns - 1), base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            u32 k1 = (u32)(base + r * Ns);
+            u64 x = v[r];
+            if (n2 && k1) x = gl::mont_mul(x, twiddle(rt, INV, (u32)(n2 * k1), logN));
+            p[((u64)E * k1) << b] = x;
+        }
+    }
+};
+
 // ---------------------------------------------------------------- step 1
 // Transforms s = (c, n2), element n1 at src[c*sstride + N2*n1 + n2]; DFT over n1
 // (inverse if INV) -> k1; times w_N^(+-n2*k1); to dst[c*dstride + E*N2*k1 + n2].
+// Adjacent threads take adjacent transforms (adjacent n2: contiguous in HBM).
 template <bool INV, int LT = LOG_TILE>
 __device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Roots& rt) {
-    constexpr int NTPB = 1 << (LT - 4);
     extern __shared__ u64 lds[];
-    const u32 logN = A.logN, a = A.a, logNT = A.logNT;
-    const u32 b = logN - a;
-    const int L = 1 << a;
-    const int Lp = padded_len(L);
-    const int NT = 1 << logNT;
-    const u64 s0 = blk << logNT;
-    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
-        int t = e & (NT - 1), i = e >> logNT;
-        u64 s = s0 + t;
-        if (s < A.ntr) {
-            u64 c = s >> b, n2 = s & ((1ull << b) - 1);
-            lds[pidx(t, i, Lp)] = A.src[c * A.sstride + ((u64)i << b) + n2];
-        }
-    }
-    __syncthreads();
-    lds_dft<INV, NTPB, 1 << LT>(lds, a, NT, rt.r4096[INV]);
-    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
-        int t = e & (NT - 1), k1 = e >> logNT;
-        u64 s = s0 + t;
-        if (s < A.ntr) {
-            u64 c = s >> b, n2 = s & ((1ull << b) - 1);
-            u64 v = lds[pidx(t, k1, Lp)];
-            if (n2 && k1) v = gl::mont_mul(v, twiddle(rt, INV, (u32)(n2 * k1), logN));
-            A.dst[c * A.dstride + (((u64)A.E * k1) << b) + n2] = v;
-        }
-    }
+    constexpr int NTPB = 1 << (LT - 4);
+    const u32 b = A.logN - A.a;
+    const u64 s0 = blk << A.logNT;
+    const Step1Load ld{A.src, A.sstride, s0, A.ntr, b};
+    const Step1Store<INV> st{A.dst, A.dstride, s0, A.ntr, b, A.E, A.logN, rt};
+    fused_dft<INV, NTPB, 1 << LT, true>(lds, A.a, 1 << A.logNT, A.logNT, rt.r4096[INV], ld, st);
 }
 
 template <bool INV, int LT = LOG_TILE>
@@ -209,13 +348,13 @@ __global__ void __launch_bounds__(1 << (LT - 4), 4) k_step1(Step1Args A, Roots r
     step1_body<INV, LT>(A, blockIdx.x, rt);
 }
 
-template <bool INV>
-__global__ void __launch_bounds__(TPB) k_step1_multi(const Step1Args* __restrict__ args,
-                                                     const u32* __restrict__ start, u32 nseg,
-                                                     Roots rt) {
+template <bool INV, int LT = LOG_TILE>
+__global__ void __launch_bounds__(1 << (LT - 4), 4) k_step1_multi(const Step1Args* __restrict__ args,
+                                                                  const u32* __restrict__ start,
+                                                                  u32 nseg, Roots rt) {
     u32 seg = find_segment(start, nseg, blockIdx.x);
     const Step1Args& A = args[seg];
-    step1_body<INV>(A, blockIdx.x - start[seg], rt);
+    step1_body<INV, LT>(A, blockIdx.x - start[seg], rt);
 }
 
 // ---------------------------------------------------------------- plain NTT step 2
@@ -358,34 +497,151 @@ __global__ void __launch_bounds__(TPB) k_lde_step2(LdeArgs A, Roots rt) {
 // expansion of k_lde_step2 on them.  One extra read and write of N elements
 // per column buys full occupancy for the inverse row DFT, which the fused
 // kernel runs with a quarter of its threads at E = 4.
-__global__ void __launch_bounds__(TPB) k_lde_rowinv(LdeArgs A, u32 logNT, Roots rt) {
+// Rows s = (c, k1) of N2 elements at src[c*scs + k1*srs + i].
+struct RowLoad {
+    const u64* src;
+    u64 scs, srs, s0, nrows;
+    u32 a;
+    template <int R>
+    __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s < nrows) {
+            const u64* p = src + (s >> a) * scs + (s & ((1ull << a) - 1)) * srs;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = p[j + r * per];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = 0;
+        }
+    }
+};
+// Row (c, k1) to out[c*M + E*N2*k1 + i].
+struct RowStore {
+    u64* out;
+    u64 M, s0, nrows;
+    u32 a, b, E;
+    template <int R>
+    __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s >= nrows) return;
+        u64* p = out + (s >> a) * M + (((u64)E * (s & ((1ull << a) - 1))) << b);
+        int k = j & (Ns - 1), base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) p[base + r * Ns] = v[r];
+    }
+};
+
+__device__ __forceinline__ void lde_rowinv_body(const LdeArgs& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
-    const u32 a = A.a, b = A.logN - A.a;
-    const int L = 1 << b;
-    const int Lp = padded_len(L);
-    const int NT = 1 << logNT;
-    const u64 nrows = A.cols << a;
-    const u64 amask = (1ull << a) - 1;
-    const u64 s0 = (u64)blockIdx.x << logNT;
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
-        int t = e >> b, i = e & (L - 1);
-        u64 s = s0 + t;
-        if (s < nrows)
-            lds[pidx(t, i, Lp)] =
-                A.src[(s >> a) * A.src_col_stride + (s & amask) * A.src_row_stride + i];
+    const u32 b = A.logN - A.a;
+    const u64 nrows = A.cols << A.a;
+    const u64 s0 = blk << A.logNT;  // rows per block of this kernel: TILE / N2
+    const RowLoad ld{A.src, A.src_col_stride, A.src_row_stride, s0, nrows, A.a};
+    const RowStore st{A.out, A.M, s0, nrows, A.a, b, A.E};
+    fused_dft<true, TPB, TILE, false>(lds, b, 1 << A.logNT, A.logNT, rt.r4096[1], ld, st);
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_rowinv(LdeArgs A, Roots rt) {
+    lde_rowinv_body(A, blockIdx.x, rt);
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_rowinv_multi(const LdeArgs* __restrict__ args,
+                                                          const u32* __restrict__ start, u32 nseg,
+                                                          Roots rt) {
+    u32 seg = find_segment(start, nseg, blockIdx.x);
+    const LdeArgs& A = args[seg];
+    lde_rowinv_body(A, blockIdx.x - start[seg], rt);
+}
+
+// Coset transforms T = sl*NT + t of row t (coefficients at src rows, written
+// by k_lde_rowinv): element k2 = (C_k2 + randomizer term) * beta_sl^(N1*k2).
+struct ExpandLoad {
+    const LdeArgs& A;
+    u64 s0, nrows;
+    u32 logNT;
+    template <int R>
+    __device__ __forceinline__ void load(int T, int j, int per, u64 (&v)[R]) const {
+        const u32 a = A.a, b = A.logN - A.a;
+        u32 sl = (u32)T >> logNT;
+        u64 s = s0 + (T & ((1 << logNT) - 1));
+        if (s >= nrows) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = 0;
+            return;
+        }
+        u64 c = s >> a, k1 = s & ((1ull << a) - 1);
+        const u64* p = A.src + c * A.src_col_stride + k1 * A.src_row_stride;
+        const u64* pb = A.pb + ((u64)sl << b);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int k2 = j + r * per;
+            u64 x = p[k2];
+            u64 rr = k1 + ((u64)k2 << a);
+            if (rr < A.rs.k) x = gl::add(x, rand_term(A, c, sl, rr));
+            v[r] = gl::mont_mul(x, pb[k2]);
+        }
     }
-    __syncthreads();
-    lds_dft<true>(lds, b, NT, rt.r4096[1]);
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
-        int t = e >> b, i = e & (L - 1);
-        u64 s = s0 + t;
-        if (s < nrows)
-            A.out[(s >> a) * A.M + (((u64)A.E * (s & amask)) << b) + i] = lds[pidx(t, i, Lp)];
+};
+// Output u1 of coset sl, row k1, times N^-1 beta_sl^k1 w_N^(k1*u1), to
+// out[c*M + E*N2*k1 + E*u1 + sl].  On the last radix-16 pass of a full row the
+// thread's outputs are u1 = j + Ns*r, so the factor advances by w_N^(k1*Ns).
+struct ExpandStore {
+    const LdeArgs& A;
+    Roots rt;
+    u64 s0, nrows;
+    u32 logNT;
+    template <int R>
+    __device__ __forceinline__ void store(int T, int j, int Ns, const u64 (&v)[R]) const {
+        const u32 a = A.a, b = A.logN - A.a, E = A.E;
+        u32 sl = (u32)T >> logNT;
+        u64 s = s0 + (T & ((1 << logNT) - 1));
+        if (s >= nrows) return;
+        u64 c = s >> a, k1 = s & ((1ull << a) - 1);
+        u64* dst = A.out + c * A.M + (((u64)E * k1) << b) + sl;
+        u64 f = A.pa[((u64)sl << a) + k1];
+        int k = j & (Ns - 1), base = (j - k) * R + k;
+        if (R * Ns == (1 << b)) {  // base = j: u1 = j + Ns*r
+            if (k1 && j) f = gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * j), A.logN));
+            const u64 step = twiddle(rt, 0, (u32)(k1 * Ns), A.logN);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                dst[(u64)E * (j + r * Ns)] = gl::mont_mul(v[r], f);
+                if (r + 1 < R) f = gl::mont_mul(f, step);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                u32 u1 = (u32)(base + r * Ns);
+                u64 g = (u1 && k1) ? gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * u1), A.logN)) : f;
+                dst[(u64)E * u1] = gl::mont_mul(v[r], g);
+            }
+        }
     }
+};
+
+// Split form, second kernel: for NT rows per block (NT * E * N2 = TILE), the E
+// coset transforms straight from the coefficient rows to the codewords.
+__device__ __forceinline__ void lde_expand_body(const LdeArgs& A, u64 blk, const Roots& rt) {
+    extern __shared__ u64 lds[];
+    const u32 b = A.logN - A.a;
+    const u64 nrows = A.cols << A.a;
+    const u64 s0 = blk << A.logNT;
+    const ExpandLoad ld{A, s0, nrows, A.logNT};
+    const ExpandStore st{A, rt, s0, nrows, A.logNT};
+    fused_dft<false, TPB, TILE, false>(lds, b, (int)(A.E << A.logNT), (int)(A.logE + A.logNT),
+                                       rt.r4096[0], ld, st);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_expand(LdeArgs A, Roots rt) {
-    lde_step2_body<false>(A, blockIdx.x, rt);
+    lde_expand_body(A, blockIdx.x, rt);
+}
+
+__global__ void __launch_bounds__(TPB) k_lde_expand_multi(const LdeArgs* __restrict__ args,
+                                                          const u32* __restrict__ start, u32 nseg,
+                                                          Roots rt) {
+    u32 seg = find_segment(start, nseg, blockIdx.x);
+    const LdeArgs& A = args[seg];
+    lde_expand_body(A, blockIdx.x - start[seg], rt);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_step2_multi(const LdeArgs* __restrict__ args,
@@ -399,36 +655,44 @@ __global__ void __launch_bounds__(TPB) k_lde_step2_multi(const LdeArgs* __restri
 // ---------------------------------------------------------------- LDE step 3
 // Transforms (c, q = u1*E + s), element k1 at out[c*M + E*N2*k1 + q]; forward
 // DFT over k1 -> u2, written back to out[c*M + E*N2*u2 + q].
+// Transforms (c, q) of one column: element i at out[c*M + (i << logQ) + q], in place.
+struct Step3IO {
+    u64* out;
+    u64 M, s0, ntr;
+    u32 logQ;
+    template <int R>
+    __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s < ntr) {
+            const u64* p = out + (s >> logQ) * M + (s & ((1ull << logQ) - 1));
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = p[(u64)(j + r * per) << logQ];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = 0;
+        }
+    }
+    template <int R>
+    __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s >= ntr) return;
+        u64* p = out + (s >> logQ) * M + (s & ((1ull << logQ) - 1));
+        int k = j & (Ns - 1), base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) p[(u64)(base + r * Ns) << logQ] = v[r];
+    }
+};
+
 template <int LT = LOG_TILE>
 __device__ __forceinline__ void lde_step3_body(const Step3Args& A, u64 blk, const Roots& rt) {
-    constexpr int NTPB = 1 << (LT - 4);
     extern __shared__ u64 lds[];
-    u64* out = A.out;
-    const u32 b = A.logN - A.a, a = A.a, logNT = A.logNT;
-    const int L = 1 << a;
-    const int Lp = padded_len(L);
-    const int NT = 1 << logNT;
+    constexpr int NTPB = 1 << (LT - 4);
+    const u32 b = A.logN - A.a;
     const u32 logQ = b + A.logE;  // transforms per column = E*N2
-    const u64 ntr = A.cols << logQ;
-    const u64 s0 = blk << logNT;
-    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
-        int t = e & (NT - 1), i = e >> logNT;
-        u64 s = s0 + t;
-        if (s < ntr) {
-            u64 c = s >> logQ, q = s & ((1ull << logQ) - 1);
-            lds[pidx(t, i, Lp)] = out[c * A.M + ((u64)i << logQ) + q];
-        }
-    }
-    __syncthreads();
-    lds_dft<false, NTPB, 1 << LT>(lds, a, NT, rt.r4096[0]);
-    for (int e = threadIdx.x; e < NT * L; e += NTPB) {
-        int t = e & (NT - 1), u2 = e >> logNT;
-        u64 s = s0 + t;
-        if (s < ntr) {
-            u64 c = s >> logQ, q = s & ((1ull << logQ) - 1);
-            out[c * A.M + ((u64)u2 << logQ) + q] = lds[pidx(t, u2, Lp)];
-        }
-    }
+    const Step3IO io{A.out, A.M, blk << A.logNT, A.cols << logQ, logQ};
+    // every input of the block is read in the first pass, before any output is
+    // written: in place is safe
+    fused_dft<false, NTPB, 1 << LT, true>(lds, A.a, 1 << A.logNT, A.logNT, rt.r4096[0], io, io);
 }
 
 template <int LT = LOG_TILE>
@@ -436,12 +700,13 @@ __global__ void __launch_bounds__(1 << (LT - 4), 4) k_lde_step3(Step3Args A, Roo
     lde_step3_body<LT>(A, blockIdx.x, rt);
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_step3_multi(const Step3Args* __restrict__ args,
-                                                         const u32* __restrict__ start, u32 nseg,
-                                                         Roots rt) {
+template <int LT = LOG_TILE>
+__global__ void __launch_bounds__(1 << (LT - 4), 4) k_lde_step3_multi(const Step3Args* __restrict__ args,
+                                                                      const u32* __restrict__ start,
+                                                                      u32 nseg, Roots rt) {
     u32 seg = find_segment(start, nseg, blockIdx.x);
     const Step3Args& A = args[seg];
-    lde_step3_body(A, blockIdx.x - start[seg], rt);
+    lde_step3_body<LT>(A, blockIdx.x - start[seg], rt);
 }
 
 // Constant tables pa/pb of one (logN, a, E, o, g).
@@ -496,7 +761,10 @@ static std::atomic<int> g_variant[kMaxLogN + 1][5];
 
 int lde_variant(u32 logN, u32 logE) {
     int v = (logN <= kMaxLogN && logE <= 4) ? g_variant[logN][logE].load() : 0;
-    return v ? v : LDE_WIDE;
+    if (v) return v;
+    // default (measured on MI355X at widths 156..379): the split form from 2^14
+    // rows on, the single fused kernel below; column tiles of 2^12
+    return (logN >= 14 ? LDE_SPLIT : LDE_WIDE) | (2 << 2);
 }
 
 bool lde_variant_valid(int v) {
@@ -600,8 +868,8 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
                 u64 m, bool truncate, u64* out, LdePlan* P, bool batch) {
     P->k1 = P->k3 = P->split = false;
     P->k2_variant = lde_variant(log2u(n), log2u(m / (n ? n : 1)));
-    if (batch && (P->k2_variant & 3) == LDE_SPLIT) P->k2_variant = LDE_WIDE;  // one kernel per phase
-    P->col_lt = batch ? LOG_TILE : LOG_TILE - (P->k2_variant >> 2);
+    // multi-table launches run the column kernels with the 2^12 tile
+    P->col_lt = batch ? kBatchColLt : LOG_TILE - (P->k2_variant >> 2);
     P->s1_blocks = P->s2_blocks = P->s3_blocks = 0;
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
@@ -672,6 +940,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
             P->inv_blocks = (u32)((nrows + (1ull << li) - 1) >> li);
             P->inv_lds = lds_bytes(b, li);
             P->s2inv = A;
+            P->s2inv.logNT = li;
             A.src = out;
             A.src_col_stride = m;
             A.src_row_stride = (u64)E << b;
@@ -710,7 +979,7 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
     }
     if (P.split) {
         hipLaunchKernelGGL(k_lde_rowinv, dim3(P.inv_blocks), dim3(TPB), P.inv_lds, st, P.s2inv,
-                           P.inv_logNT, ds->roots);
+                           ds->roots);
         if ((rc = check_launch())) return rc;
         hipLaunchKernelGGL(k_lde_expand, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2,
                            ds->roots);
@@ -738,15 +1007,28 @@ int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg
     if (blocks == 0) return TVH_OK;
     DeviceState* ds = current_device();
     if (!ds) return TVH_ERR_DEVICE;
-    if (phase == 1)
-        hipLaunchKernelGGL(k_step1_multi<true>, dim3(blocks), dim3(TPB), lds, st,
-                           (const Step1Args*)d_args, d_start, nseg, ds->roots);
-    else if (phase == 2)
-        hipLaunchKernelGGL(k_lde_step2_multi, dim3(blocks), dim3(TPB), lds, st,
-                           (const LdeArgs*)d_args, d_start, nseg, ds->roots);
-    else
-        hipLaunchKernelGGL(k_lde_step3_multi, dim3(blocks), dim3(TPB), lds, st,
-                           (const Step3Args*)d_args, d_start, nseg, ds->roots);
+    const dim3 ctpb(1u << (kBatchColLt - 4));
+    switch (phase) {
+        case LDE_PHASE_STEP1:
+            hipLaunchKernelGGL((k_step1_multi<true, kBatchColLt>), dim3(blocks), ctpb, lds, st,
+                               (const Step1Args*)d_args, d_start, nseg, ds->roots);
+            break;
+        case LDE_PHASE_ROWINV:
+            hipLaunchKernelGGL(k_lde_rowinv_multi, dim3(blocks), dim3(TPB), lds, st,
+                               (const LdeArgs*)d_args, d_start, nseg, ds->roots);
+            break;
+        case LDE_PHASE_STEP2:
+            hipLaunchKernelGGL(k_lde_step2_multi, dim3(blocks), dim3(TPB), lds, st,
+                               (const LdeArgs*)d_args, d_start, nseg, ds->roots);
+            break;
+        case LDE_PHASE_EXPAND:
+            hipLaunchKernelGGL(k_lde_expand_multi, dim3(blocks), dim3(TPB), lds, st,
+                               (const LdeArgs*)d_args, d_start, nseg, ds->roots);
+            break;
+        default:
+            hipLaunchKernelGGL(k_lde_step3_multi<kBatchColLt>, dim3(blocks), ctpb, lds, st,
+                               (const Step3Args*)d_args, d_start, nseg, ds->roots);
+    }
     return check_launch();
 }
 
@@ -813,9 +1095,11 @@ int ntt_set_attributes() {
                          (const void*)k_ntt_step2<true>, (const void*)k_ntt_step2<false>,
                          (const void*)k_lde_step2, (const void*)k_lde_step3<14>,
                          (const void*)k_lde_step3<13>, (const void*)k_lde_step3<12>,
-                         (const void*)k_step1_multi<true>, (const void*)k_lde_step2_multi,
-                         (const void*)k_lde_step3_multi, (const void*)k_lde_rowinv,
-                         (const void*)k_lde_expand,
+                         (const void*)k_step1_multi<true, kBatchColLt>,
+                         (const void*)k_lde_step2_multi,
+                         (const void*)k_lde_step3_multi<kBatchColLt>, (const void*)k_lde_rowinv,
+                         (const void*)k_lde_expand, (const void*)k_lde_rowinv_multi,
+                         (const void*)k_lde_expand_multi,
 };
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)

@@ -125,7 +125,7 @@ struct LdePlan {
     int k2_variant;  // full variant code (form | tile << 2)
     bool split;      // LDE_SPLIT: s2inv (k_lde_rowinv) runs before s2 (k_lde_expand)
     LdeArgs s2inv;
-    u32 inv_logNT, inv_blocks;
+    u32 inv_logNT, inv_blocks;  // s2inv.logNT == inv_logNT
     size_t inv_lds;
     u32 col_lt;      // log2 tile of the column kernels (K1, K3)
     Step1Args s1;
@@ -134,10 +134,19 @@ struct LdePlan {
     u32 s1_blocks, s2_blocks, s3_blocks;
     size_t s1_lds, s2_lds, s3_lds;
 };
-// batch = true keeps the column kernels at the 2^14 tile of the multi-table kernels.
+// batch = true plans the column kernels with the multi-table tile (kBatchColLt).
 int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
                 u64 m, bool truncate, u64* out, LdePlan* plan, bool batch);
-// phase 1/2/3: k_step1_multi<true> / k_lde_step2_multi / k_lde_step3_multi
+// Phases of the multi-table LDE, in launch order.
+enum LdePhase {
+    LDE_PHASE_STEP1 = 0,   // k_step1_multi<true>
+    LDE_PHASE_ROWINV = 1,  // k_lde_rowinv_multi (split-form tables)
+    LDE_PHASE_STEP2 = 2,   // k_lde_step2_multi (wide-form tables)
+    LDE_PHASE_EXPAND = 3,  // k_lde_expand_multi (split-form tables)
+    LDE_PHASE_STEP3 = 4,   // k_lde_step3_multi
+    LDE_PHASES = 5
+};
+constexpr u32 kBatchColLt = 12;  // column-kernel tile of multi-table launches
 int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg, u32 blocks,
                      size_t lds, hipStream_t st);
 int launch_fill_uniform(u64* d, u64 count, u64 seed, hipStream_t st);
