@@ -56,6 +56,17 @@ int main() {
 #undef C4
     if (e != 96) bad++;
   }
+  // lin52: L + H * 2^32 mod p for L, H < 2^52 (edges and random)
+  {
+    const uint64_t M52 = (1ull << 52) - 1;
+    uint64_t ed[] = {0, 1, 0xFFFFFFFFull, 0x100000000ull, M52, M52 - 1, 0xFFFFF00000000ull, 0xFFFFFFFFFFFFFull,
+                     0x100000000000ull, 0xFFFFFFFF0ull};
+    for (int i = 0; i < 200000; i++) {
+      uint64_t L = i < 100 ? ed[i % 10] : (rng() & M52), H = i < 100 ? ed[i / 10] : (rng() & M52);
+      uint64_t v = gl::lin52(L, H);
+      if (v % P != (uint64_t)(((u128)L + ((u128)H << 32)) % P)) bad++;
+    }
+  }
   for (int e = 0; e < 192; e++) { u128 r = 1; for (int i=0;i<e;i++) r = (r*2)%P; if (gl::pow2_mod_p(e) != (uint64_t)r) bad++; }
   printf("bad=%d\n", bad);
   return bad != 0;

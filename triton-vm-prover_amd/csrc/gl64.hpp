@@ -155,6 +155,19 @@ __host__ __device__ __forceinline__ u64 montyred(u64 lo, u64 hi) {
     return mk64(r0, r1);
 }
 
+// L + H * 2^32 mod p for L, H < 2^52 (the MDS recombination) -> < 2^64, in
+// 32-bit words: with 2^64 = 2^32 - 1 mod p the value is (L0 - H1) +
+// (L1 + H0 + H1) * 2^32; a carry out of the high word is folded back as 2^32 - 1.
+__host__ __device__ __forceinline__ u64 lin52(u64 L, u64 H) {
+    u32 L0 = (u32)L, L1 = (u32)(L >> 32), H0 = (u32)H, H1 = (u32)(H >> 32);
+    u32 b;
+    u32 lo = __builtin_subc(L0, H1, 0u, &b);
+    u64 hi = (u64)H0 + (L1 + H1) - b;  // < 2^32 + 2^21, never negative (b = 1 needs H1 > 0)
+    u64 v = mk64(lo, (u32)hi);
+    if (hi >> 32) v += EPS;  // v < 2^53 here: no second carry
+    return v;
+}
+
 // a * a * 2^-64 mod p, any u64 a -> < 2^64
 __host__ __device__ __forceinline__ u64 mont_sqr(u64 a) {
     u64 lo, hi;

@@ -239,13 +239,23 @@ __device__ __forceinline__ void fused_passes(const LdsIO& io, int NT, int logNT,
      ...);
 }
 
+// LDS row pitch of a transform for the fused DFT: L + L/16 + pad, with pad
+// chosen so that the lanes of a 16-lane LDS cycle hit distinct banks under the
+// transform-fastest mapping (lane = j * NT' + t, NT' = min(NT, 16)):
+// t * pitch + j is then distinct mod 16 for t < NT', j < 16 / NT'.
+__host__ __device__ inline int fused_pitch(int L, int NT, bool tf) {
+    int ntp = NT < 16 ? NT : 16;
+    return L + (L >> 4) + (tf ? 16 / ntp : 1);
+}
+
 // NT (power of two) transforms of length 2^logL (<= 4096), NT * L <= NTILE.
-// The block's LDS (buf) must not be in use by other threads on entry.
+// The block's LDS (buf, NT * fused_pitch(L, NT, TF) u64) must not be in use by
+// other threads on entry.
 template <bool INV, int NTPB, int NTILE, bool TF, class Ld, class St>
 __device__ __forceinline__ void fused_dft(u64* buf, int logL, int NT, int logNT,
                                           const u64* __restrict__ r4096, const Ld& ld,
                                           const St& st) {
-    const LdsIO io{buf, padded_len(1 << logL)};
+    const LdsIO io{buf, fused_pitch(1 << logL, NT, TF)};
 #define TVH_FUSED(q, r)                                                                      \
     fused_passes<q, r, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st,                 \
                                              std::make_index_sequence<(q) + ((r) != 0 || (q) == 0)>{})
@@ -553,17 +563,19 @@ __global__ void __launch_bounds__(TPB) k_lde_rowinv_multi(const LdeArgs* __restr
     lde_rowinv_body(A, blockIdx.x - start[seg], rt);
 }
 
-// Coset transforms T = sl*NT + t of row t (coefficients at src rows, written
+// Coset transforms T = t*E + sl of row t (coefficients at src rows, written
 // by k_lde_rowinv): element k2 = (C_k2 + randomizer term) * beta_sl^(N1*k2).
+// Cosets are the fastest index, so that with the transform-fastest thread
+// mapping the E cosets of one output position are written by adjacent lanes
+// (out[E*u1 + sl] contiguous).
 struct ExpandLoad {
     const LdeArgs& A;
     u64 s0, nrows;
-    u32 logNT;
     template <int R>
     __device__ __forceinline__ void load(int T, int j, int per, u64 (&v)[R]) const {
         const u32 a = A.a, b = A.logN - A.a;
-        u32 sl = (u32)T >> logNT;
-        u64 s = s0 + (T & ((1 << logNT) - 1));
+        u32 sl = (u32)T & (A.E - 1);
+        u64 s = s0 + ((u32)T >> A.logE);
         if (s >= nrows) {
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = 0;
@@ -589,12 +601,11 @@ struct ExpandStore {
     const LdeArgs& A;
     Roots rt;
     u64 s0, nrows;
-    u32 logNT;
     template <int R>
     __device__ __forceinline__ void store(int T, int j, int Ns, const u64 (&v)[R]) const {
         const u32 a = A.a, b = A.logN - A.a, E = A.E;
-        u32 sl = (u32)T >> logNT;
-        u64 s = s0 + (T & ((1 << logNT) - 1));
+        u32 sl = (u32)T & (E - 1);
+        u64 s = s0 + ((u32)T >> A.logE);
         if (s >= nrows) return;
         u64 c = s >> a, k1 = s & ((1ull << a) - 1);
         u64* dst = A.out + c * A.M + (((u64)E * k1) << b) + sl;
@@ -626,10 +637,10 @@ __device__ __forceinline__ void lde_expand_body(const LdeArgs& A, u64 blk, const
     const u32 b = A.logN - A.a;
     const u64 nrows = A.cols << A.a;
     const u64 s0 = blk << A.logNT;
-    const ExpandLoad ld{A, s0, nrows, A.logNT};
-    const ExpandStore st{A, rt, s0, nrows, A.logNT};
-    fused_dft<false, TPB, TILE, false>(lds, b, (int)(A.E << A.logNT), (int)(A.logE + A.logNT),
-                                       rt.r4096[0], ld, st);
+    const ExpandLoad ld{A, s0, nrows};
+    const ExpandStore st{A, rt, s0, nrows};
+    fused_dft<false, TPB, TILE, true>(lds, b, (int)(A.E << A.logNT), (int)(A.logE + A.logNT),
+                                      rt.r4096[0], ld, st);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_expand(LdeArgs A, Roots rt) {
@@ -898,7 +909,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
         u64 ntr = cols << b;
         P->s1 = Step1Args{trace, n, out, m, logN, a, E, ntr, logNT};
         P->s1_blocks = (u32)((ntr + (1ull << logNT) - 1) >> logNT);
-        P->s1_lds = lds_bytes(a, logNT);
+        P->s1_lds = (size_t)fused_pitch(1 << a, 1 << logNT, true) * (1u << logNT) * sizeof(u64);
     }
     // K2: INTT step 2 + coset scaling + forward step 1
     {
@@ -932,6 +943,8 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
         A.src_row_stride = P->k1 ? ((u64)E << b) : 0;
         P->s2_blocks = (u32)((nrows + (1ull << logNT) - 1) >> logNT);
         P->s2_lds = lds_bytes(b, logNT + logE);
+        if ((P->k2_variant & 3) == LDE_SPLIT)  // k_lde_expand: transform-fastest pitch
+            P->s2_lds = (size_t)fused_pitch(1 << b, (int)(E << logNT), true) * (E << logNT) * sizeof(u64);
         P->split = (P->k2_variant & 3) == LDE_SPLIT;
         if (P->split) {  // inverse rows first (TILE / N2 rows per block), then expansion from out
             u32 li = tile_log_nt(b);
@@ -953,7 +966,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
         u64 ntr = cols << (b + logE);
         P->s3 = Step3Args{out, logN, a, logE, cols, m, logNT};
         P->s3_blocks = (u32)((ntr + (1ull << logNT) - 1) >> logNT);
-        P->s3_lds = lds_bytes(a, logNT);
+        P->s3_lds = (size_t)fused_pitch(1 << a, 1 << logNT, true) * (1u << logNT) * sizeof(u64);
     }
     return TVH_OK;
 }
@@ -1061,12 +1074,14 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st)
             u64 ntr = g << b;
             u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
             Step1Args A{d, n, scratch, n, logN, a, 1u, ntr, logNT};
+            const size_t s1_lds =
+                (size_t)fused_pitch(1 << a, 1 << logNT, true) * (1u << logNT) * sizeof(u64);
             if (inverse)
-                hipLaunchKernelGGL(k_step1<true>, dim3((u32)blocks), dim3(TPB),
-                                   lds_bytes(a, logNT), st, A, ds->roots);
+                hipLaunchKernelGGL(k_step1<true>, dim3((u32)blocks), dim3(TPB), s1_lds, st, A,
+                                   ds->roots);
             else
-                hipLaunchKernelGGL(k_step1<false>, dim3((u32)blocks), dim3(TPB),
-                                   lds_bytes(a, logNT), st, A, ds->roots);
+                hipLaunchKernelGGL(k_step1<false>, dim3((u32)blocks), dim3(TPB), s1_lds, st, A,
+                                   ds->roots);
             if ((rc = check_launch())) return rc;
         }
         {

@@ -154,15 +154,7 @@ __device__ __forceinline__ void mds(u64 s[STATE]) {
     mds_half(hi, yhi);
 #pragma unroll
     for (int r = 0; r < STATE; r++) {
-        u64 L = f2u52(ylo[r]), H = f2u52(yhi[r]);
-        // value = L + H * 2^32, H = Hhi * 2^32 + Hlo (Hhi < 2^20)
-        u64 Hhi = H >> 32;
-        u64 t = L + (H << 32);
-        if (t < L) t += gl::EPS;
-        u64 u = (Hhi << 32) - Hhi;  // Hhi * 2^64 = Hhi * (2^32 - 1)
-        u64 v = t + u;
-        if (v < u) v += gl::EPS;
-        s[r] = v;
+        s[r] = gl::lin52(f2u52(ylo[r]), f2u52(yhi[r]));  // L + H * 2^32
     }
 }
 

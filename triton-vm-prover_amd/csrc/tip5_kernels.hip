@@ -383,6 +383,11 @@ int merkle_groups(u64* tree, u64 n, int layout, std::vector<MerkleGroup>& groups
         u32 T = parents >= 256 ? 256 : (u32)parents;  // parents is a power of two
         u32 maxlev = 0;
         while ((1u << maxlev) <= T) maxlev++;           // T=256 -> 9 levels
+        // Large levels get a launch each: continuing inside a block halves the
+        // active threads per level, so a 9-level group keeps the GPU ~22 % busy.
+        // Only the top levels (<= 2^16 parents), which cannot fill the GPU
+        // anyway, are built as in-block subtrees.
+        if (parents > (1u << 16)) maxlev = 1;
         u32 levels = logn - lvl < maxlev ? logn - lvl : maxlev;
         MerkleGroup g;
         g.args.src = tree + 5 * merkle_level_base(n, lvl, layout);
