@@ -432,6 +432,14 @@ def autotune(args, lib, shapes):
     return out
 
 
+def lde_kernels(lib, n, m) -> str:
+    """Names the kernels of the LDE variant in use for (n, m)."""
+    v = lib.lde_get_variant(n.bit_length() - 1, (m // n).bit_length() - 1)
+    tile = 14 - (v >> 2)
+    row = "k_lde_rowinv + k_lde_expand" if v & 3 == 2 else "k_lde_step2"
+    return f"LDE passes (k_step1<{tile}> + {row} + k_lde_step3<{tile}>)"
+
+
 def workload_name(args, n, w, E) -> str:
     lg, lm = args.log_n, args.log_n + E.bit_length() - 1
     if args.mode == "aux":
@@ -592,7 +600,7 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(args.log_n, w, E),
-                     "kernel": "LDE passes (k_step1 + k_lde_step2 + k_lde_step3)",
+                     "kernel": lde_kernels(lib, n, m),
                      "algorithmic_bytes": lde_bytes},
         "lde_forms": tuned,
         "tip5": {"permutations": perms,
