@@ -435,8 +435,10 @@ def autotune(args, lib, shapes):
 def lde_kernels(lib, n, m) -> str:
     """Names the kernels of the LDE variant in use for (n, m)."""
     v = lib.lde_get_variant(n.bit_length() - 1, (m // n).bit_length() - 1)
-    tile = 14 - (v >> 2)
+    tile = 14 - ((v >> 2) & 3)
     row = "k_lde_rowinv + k_lde_expand" if v & 3 == 2 else "k_lde_step2"
+    if v & 16:
+        row = "k_lde_rowinv<13> + k_lde_expand<13>"
     return f"LDE passes (k_step1<{tile}> + {row} + k_lde_step3<{tile}>)"
 
 

@@ -1,5 +1,6 @@
 """Every LDE kernel variant (row-step form TVH_LDE_WIDE / TVH_LDE_SPLIT times the
-column-step tile 2^14 / 2^13 / 2^12) against the oracle on every shape class (rows
+column-step tile 2^14 / 2^13 / 2^12, and the split form with 2^13-element row
+tiles) against the oracle on every shape class (rows
 shorter than, equal to and split from 4096; expansions 1..16; randomizers; trace
 offsets), in single-table calls and inside batch plans, and the autotuner's contract."""
 import numpy as np
@@ -9,8 +10,8 @@ from test_gpu_parity import LDE_CASES
 
 pytestmark = pytest.mark.gpu
 
-WIDE, SPLIT = 1, 2
-VARIANTS = (1, 2, 5, 6, 9, 10)  # form | tile code << 2
+WIDE, SPLIT, HALF = 1, 2, 16
+VARIANTS = (1, 2, 5, 6, 9, 10, 18, 22, 26)  # form | tile code << 2 | half row tile
 EXTRA = [(20, 4, 0, 1), (18, 2, 5, 3), (17, 16, 0, 1), (19, 1, 0, 1)]
 
 
@@ -73,7 +74,8 @@ def test_lde_forms_agree_large(T, oracle, variant, logn, E, k, toff):
 def test_batch_plan_with_both_forms(T, oracle, variant):
     """A batch whose tables use different forms equals per-table commits."""
     shapes = [(10, 5, 4, WIDE), (12, 3, 4, SPLIT), (13, 7, 2, SPLIT | 4), (8, 11, 8, WIDE | 8),
-              (14, 2, 4, SPLIT | 8), (11, 4, 4, SPLIT), (15, 3, 2, WIDE | 4)]
+              (14, 2, 4, SPLIT | 8), (11, 4, 4, SPLIT), (15, 3, 2, WIDE | 4),
+              (16, 2, 4, SPLIT | HALF), (9, 3, 16, SPLIT | 4 | HALF)]
     tabs = []
     for i, (lg, w, E, form) in enumerate(shapes):
         variant(lg, E, form)
@@ -89,6 +91,19 @@ def test_batch_plan_with_both_forms(T, oracle, variant):
         root, cw, _ = oracle.commit(x, t["n"], w, t["m"], want_codewords=True)
         assert np.array_equal(T.host(t["cw"]), cw), (lg, w, E, form)
         assert np.array_equal(T.host(t["tree"]).reshape(-1, 5)[-1], root)
+
+
+def test_variant_codes(T, tvhmod):
+    """The library accepts exactly the listed codes (and 0, the default)."""
+    assert T.lib.lib.tvh_lde_variant_count() == len(VARIANTS)
+    for v in range(32):
+        ok = v == 0 or v in VARIANTS
+        if ok:
+            T.lib.lde_set_variant(13, 2, v)
+        else:
+            with pytest.raises(tvhmod.TVHError):
+                T.lib.lde_set_variant(13, 2, v)
+    T.lib.lde_set_variant(13, 2, 0)
 
 
 def test_autotune_contract(T, variant):

@@ -203,7 +203,10 @@ int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant) {
     return set_lde_variant(log_n, log_e, variant);
 }
 
-int tvh_lde_variant_count(void) { return 6; }
+static const int kLdeVariants[] = {1, 2, 5, 6, 9, 10, 18, 22, 26};  // form | tile << 2 | half
+static constexpr int kNumLdeVariants = sizeof(kLdeVariants) / sizeof(kLdeVariants[0]);
+
+int tvh_lde_variant_count(void) { return kNumLdeVariants; }
 
 int tvh_lde_get_variant(uint32_t log_n, uint32_t log_e, int* variant) {
     if (!variant) return set_error(TVH_ERR_INVALID_ARG, "null variant pointer");
@@ -234,12 +237,11 @@ int tvh_lde_autotune(size_t cols, size_t n, size_t m, int* chosen, float* ms) {
                 hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = set_error(TVH_ERR_DEVICE, "autotune stream/events");
     if (!rc) rc = launch_fill_uniform(trace, n * tc, 77, st);
-    const int variants[6] = {1, 2, 5, 6, 9, 10};  // form | tile << 2 (tiles 2^14, 2^13, 2^12)
     float best = 0;
     int best_v = LDE_WIDE;
     int prev = lde_variant(logN, logE);
-    for (int i = 0; i < 6 && !rc; i++) {
-        set_lde_variant(logN, logE, variants[i]);
+    for (int i = 0; i < kNumLdeVariants && !rc; i++) {
+        set_lde_variant(logN, logE, kLdeVariants[i]);
         RandSpec none;
         rc = launch_lde(trace, tc, n, none, 1, 7, m, false, out, st);  // warm-up
         if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = set_error(TVH_ERR_DEVICE, "event");
@@ -252,7 +254,7 @@ int tvh_lde_autotune(size_t cols, size_t n, size_t m, int* chosen, float* ms) {
         if (ms) ms[i] = t;
         if (!rc && (i == 0 || t < best)) {
             best = t;
-            best_v = variants[i];
+            best_v = kLdeVariants[i];
         }
     }
     set_lde_variant(logN, logE, rc ? prev : best_v);

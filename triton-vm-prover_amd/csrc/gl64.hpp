@@ -94,13 +94,14 @@ __host__ __device__ __forceinline__ void mul_wide_4mad(u64 a, u64 b, u64& lo, u6
     u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
     u64 p00 = (u64)a0 * b0;
     u64 q = (u64)a1 * b1;
-    u64 S;
-    bool c = __builtin_add_overflow((u64)a1 * b0, (u64)a0 * b1, &S);
-    u32 k1, k2, k3, k4;
-    u32 l1 = __builtin_addc((u32)(p00 >> 32), (u32)S, 0u, &k1);
-    u32 h0 = __builtin_addc((u32)q, (u32)(S >> 32), k1, &k2);
+    u64 x = (u64)a1 * b0, y = (u64)a0 * b1;
+    u32 k0, c, k1, k2, k3, k4;
+    u32 S0 = __builtin_addc((u32)x, (u32)y, 0u, &k0);  // cross sum in words: the 64-bit
+    u32 S1 = __builtin_addc((u32)(x >> 32), (u32)(y >> 32), k0, &c);  // add + compare costs more
+    u32 l1 = __builtin_addc((u32)(p00 >> 32), S0, 0u, &k1);
+    u32 h0 = __builtin_addc((u32)q, S1, k1, &k2);
     u32 h1 = __builtin_addc((u32)(q >> 32), 0u, k2, &k3);
-    h1 = __builtin_addc(h1, 0u, (u32)c, &k4);
+    h1 = __builtin_addc(h1, 0u, c, &k4);
     lo = mk64((u32)p00, l1);
     hi = mk64(h0, h1);
 }
@@ -160,12 +161,18 @@ __host__ __device__ __forceinline__ u64 montyred(u64 lo, u64 hi) {
 // (L1 + H0 + H1) * 2^32; a carry out of the high word is folded back as 2^32 - 1.
 __host__ __device__ __forceinline__ u64 lin52(u64 L, u64 H) {
     u32 L0 = (u32)L, L1 = (u32)(L >> 32), H0 = (u32)H, H1 = (u32)(H >> 32);
-    u32 b;
+    u32 b, b2, c, c2, d, d2;
     u32 lo = __builtin_subc(L0, H1, 0u, &b);
-    u64 hi = (u64)H0 + (L1 + H1) - b;  // < 2^32 + 2^21, never negative (b = 1 needs H1 > 0)
-    u64 v = mk64(lo, (u32)hi);
-    if (hi >> 32) v += EPS;  // v < 2^53 here: no second carry
-    return v;
+    u32 t = __builtin_subc(L1 + H1, 0u, b, &b2);  // L1 + H1 - b >= 0: b = 1 needs H1 > 0
+    u32 h = __builtin_addc(H0, t, 0u, &c);       // high word and its carry (value 2^64)
+    // carry: add 2^64 = 2^32 - 1 (mod p); then h < 2^21, so h + 1 cannot overflow
+    lo = __builtin_subc(lo, 0u, c, &d);
+    h = __builtin_addc(h, 0u, c, &c2);
+    h = __builtin_subc(h, 0u, d, &d2);
+    (void)b2;
+    (void)c2;
+    (void)d2;
+    return mk64(lo, h);
 }
 
 // a * a * 2^-64 mod p, any u64 a -> < 2^64

@@ -541,18 +541,23 @@ struct RowStore {
     }
 };
 
+// Row tiles of 2^LT elements, 16 per thread (LT = 14: 1024 threads, one block
+// per CU; LT = 13: 512 threads, two blocks per CU).
+template <int LT = LOG_TILE>
 __device__ __forceinline__ void lde_rowinv_body(const LdeArgs& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
     const u32 b = A.logN - A.a;
     const u64 nrows = A.cols << A.a;
-    const u64 s0 = blk << A.logNT;  // rows per block of this kernel: TILE / N2
+    const u64 s0 = blk << A.logNT;  // rows per block of this kernel: 2^LT / N2
     const RowLoad ld{A.src, A.src_col_stride, A.src_row_stride, s0, nrows, A.a};
     const RowStore st{A.out, A.M, s0, nrows, A.a, b, A.E};
-    fused_dft<true, TPB, TILE, false>(lds, b, 1 << A.logNT, A.logNT, rt.r4096[1], ld, st);
+    fused_dft<true, (1 << (LT - 4)), (1 << LT), false>(lds, b, 1 << A.logNT, A.logNT,
+                                                        rt.r4096[1], ld, st);
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_rowinv(LdeArgs A, Roots rt) {
-    lde_rowinv_body(A, blockIdx.x, rt);
+template <int LT>
+__global__ void __launch_bounds__(1 << (LT - 4)) k_lde_rowinv(LdeArgs A, Roots rt) {
+    lde_rowinv_body<LT>(A, blockIdx.x, rt);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_rowinv_multi(const LdeArgs* __restrict__ args,
@@ -630,8 +635,9 @@ struct ExpandStore {
     }
 };
 
-// Split form, second kernel: for NT rows per block (NT * E * N2 = TILE), the E
+// Split form, second kernel: for NT rows per block (NT * E * N2 = 2^LT), the E
 // coset transforms straight from the coefficient rows to the codewords.
+template <int LT = LOG_TILE>
 __device__ __forceinline__ void lde_expand_body(const LdeArgs& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
     const u32 b = A.logN - A.a;
@@ -639,12 +645,13 @@ __device__ __forceinline__ void lde_expand_body(const LdeArgs& A, u64 blk, const
     const u64 s0 = blk << A.logNT;
     const ExpandLoad ld{A, s0, nrows};
     const ExpandStore st{A, rt, s0, nrows};
-    fused_dft<false, TPB, TILE, true>(lds, b, (int)(A.E << A.logNT), (int)(A.logE + A.logNT),
-                                      rt.r4096[0], ld, st);
+    fused_dft<false, (1 << (LT - 4)), (1 << LT), true>(
+        lds, b, (int)(A.E << A.logNT), (int)(A.logE + A.logNT), rt.r4096[0], ld, st);
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_expand(LdeArgs A, Roots rt) {
-    lde_expand_body(A, blockIdx.x, rt);
+template <int LT>
+__global__ void __launch_bounds__(1 << (LT - 4)) k_lde_expand(LdeArgs A, Roots rt) {
+    lde_expand_body<LT>(A, blockIdx.x, rt);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_expand_multi(const LdeArgs* __restrict__ args,
@@ -779,8 +786,9 @@ int lde_variant(u32 logN, u32 logE) {
 }
 
 bool lde_variant_valid(int v) {
-    int form = v & 3, tile = v >> 2;
-    return v == LDE_AUTO || ((form == LDE_WIDE || form == LDE_SPLIT) && tile <= 2);
+    int form = v & 3, tile = (v >> 2) & 3, half = v >> 4;
+    return v == LDE_AUTO ||
+           ((form == LDE_WIDE || form == LDE_SPLIT) && tile <= 2 && half <= (form == LDE_SPLIT));
 }
 
 int set_lde_variant(u32 logN, u32 logE, int v) {
@@ -790,12 +798,12 @@ int set_lde_variant(u32 logN, u32 logE, int v) {
     return TVH_OK;
 }
 
-static u32 choose_split(u32 logN, u32 E) {
-    // N2 (the contiguous row step) as long as one tile allows for all E cosets
-    // (E * N2 <= TILE, N2 <= 4096); N1 = N / N2 takes the strided step, whose
-    // HBM segments are TILE / N1 elements wide.
+static u32 choose_split(u32 logN, u32 E, u32 row_lt = LOG_TILE) {
+    // N2 (the contiguous row step) as long as one row tile allows for all E
+    // cosets (E * N2 <= 2^row_lt, N2 <= 4096); N1 = N / N2 takes the strided
+    // step, whose HBM segments are (column tile) / N1 elements wide.
     u32 logE = log2u(E);
-    u32 bmax = LOG_TILE - logE;
+    u32 bmax = row_lt - logE;
     if (bmax > 12) bmax = 12;
     u32 b = logN < bmax ? logN : bmax;
     return logN - b;
@@ -879,8 +887,10 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
                 u64 m, bool truncate, u64* out, LdePlan* P, bool batch) {
     P->k1 = P->k3 = P->split = false;
     P->k2_variant = lde_variant(log2u(n), log2u(m / (n ? n : 1)));
-    // multi-table launches run the column kernels with the 2^12 tile
-    P->col_lt = batch ? kBatchColLt : LOG_TILE - (P->k2_variant >> 2);
+    // multi-table launches run the column kernels with the 2^12 tile and the
+    // row kernels with the full tile
+    P->col_lt = batch ? kBatchColLt : LOG_TILE - ((P->k2_variant >> 2) & 3);
+    P->row_lt = (!batch && (P->k2_variant & LDE_ROW_HALF)) ? LOG_TILE - 1 : LOG_TILE;
     P->s1_blocks = P->s2_blocks = P->s3_blocks = 0;
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
@@ -894,7 +904,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
     if (logN > 23) return set_error(TVH_ERR_UNSUPPORTED, "trace_len > 2^23");
     DeviceState* ds = current_device();
     if (!ds) return TVH_ERR_DEVICE;
-    u32 a = choose_split(logN, E);
+    u32 a = choose_split(logN, E, P->row_lt);
     u32 b = logN - a;
     if (a > 12) return set_error(TVH_ERR_UNSUPPORTED, "split");
     ConstVal cv;
@@ -933,8 +943,8 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
             A.delta_n[s] = gl::mul(d, nmod);
             ws = gl::mul(ws, wE);
         }
-        u32 logNT = LOG_TILE - (b + logE);  // NT * E * N2 = TILE
-        if (b < 5) logNT = LOG_TILE - 5 - logE;
+        u32 logNT = P->row_lt - (b + logE);  // NT * E * N2 = 2^row_lt
+        if (b < 5) logNT = P->row_lt - 5 - logE;
         u64 nrows = cols << a;
         if ((1ull << logNT) > nrows) logNT = log2u(nrows);
         A.logNT = logNT;
@@ -947,7 +957,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
             P->s2_lds = (size_t)fused_pitch(1 << b, (int)(E << logNT), true) * (E << logNT) * sizeof(u64);
         P->split = (P->k2_variant & 3) == LDE_SPLIT;
         if (P->split) {  // inverse rows first (TILE / N2 rows per block), then expansion from out
-            u32 li = tile_log_nt(b);
+            u32 li = tile_log_nt(b, P->row_lt);
             if ((1ull << li) > nrows) li = log2u(nrows);
             P->inv_logNT = li;
             P->inv_blocks = (u32)((nrows + (1ull << li) - 1) >> li);
@@ -991,11 +1001,20 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
         if ((rc = check_launch())) return rc;
     }
     if (P.split) {
-        hipLaunchKernelGGL(k_lde_rowinv, dim3(P.inv_blocks), dim3(TPB), P.inv_lds, st, P.s2inv,
-                           ds->roots);
+        const dim3 rtpb(1u << (P.row_lt - 4));
+        if (P.row_lt == LOG_TILE)
+            hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE>, dim3(P.inv_blocks), rtpb, P.inv_lds, st,
+                               P.s2inv, ds->roots);
+        else
+            hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE - 1>, dim3(P.inv_blocks), rtpb, P.inv_lds, st,
+                               P.s2inv, ds->roots);
         if ((rc = check_launch())) return rc;
-        hipLaunchKernelGGL(k_lde_expand, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2,
-                           ds->roots);
+        if (P.row_lt == LOG_TILE)
+            hipLaunchKernelGGL(k_lde_expand<LOG_TILE>, dim3(P.s2_blocks), rtpb, P.s2_lds, st, P.s2,
+                               ds->roots);
+        else
+            hipLaunchKernelGGL(k_lde_expand<LOG_TILE - 1>, dim3(P.s2_blocks), rtpb, P.s2_lds, st,
+                               P.s2, ds->roots);
     } else
         hipLaunchKernelGGL(k_lde_step2, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2,
                            ds->roots);
@@ -1112,8 +1131,9 @@ int ntt_set_attributes() {
                          (const void*)k_lde_step3<13>, (const void*)k_lde_step3<12>,
                          (const void*)k_step1_multi<true, kBatchColLt>,
                          (const void*)k_lde_step2_multi,
-                         (const void*)k_lde_step3_multi<kBatchColLt>, (const void*)k_lde_rowinv,
-                         (const void*)k_lde_expand, (const void*)k_lde_rowinv_multi,
+                         (const void*)k_lde_step3_multi<kBatchColLt>, (const void*)k_lde_rowinv<LOG_TILE>,
+                         (const void*)k_lde_expand<LOG_TILE>, (const void*)k_lde_rowinv<LOG_TILE - 1>,
+                         (const void*)k_lde_expand<LOG_TILE - 1>, (const void*)k_lde_rowinv_multi,
                          (const void*)k_lde_expand_multi,
 };
     for (const void* f : fns)

@@ -114,20 +114,24 @@ struct LdeArgs {
 // LDE kernel variant: bits 0-1 the form of the fused row step (LDE_WIDE: one
 // kernel, all E cosets of a row in one 1024-thread LDS tile; LDE_SPLIT: the
 // inverse row DFTs as a kernel of their own, then the expansion); bits 2-3 the
-// column kernels' tile, 2^(14 - code) elements with 16 per thread.
-enum LdeVariant { LDE_AUTO = 0, LDE_WIDE = 1, LDE_SPLIT = 2 };
+// column kernels' tile, 2^(14 - code) elements with 16 per thread; bit 4
+// (LDE_SPLIT only) halves the row kernels' tile to 2^13 elements (512 threads,
+// two blocks per CU), which also halves the row length N2 when E * N2 would
+// not fit.
+enum LdeVariant { LDE_AUTO = 0, LDE_WIDE = 1, LDE_SPLIT = 2, LDE_ROW_HALF = 16 };
 int lde_variant(u32 logN, u32 logE);            // the tuned choice (never LDE_AUTO)
 bool lde_variant_valid(int v);
 int set_lde_variant(u32 logN, u32 logE, int v);  // v = LDE_AUTO restores the default
 
 struct LdePlan {
     bool k1, k3;
-    int k2_variant;  // full variant code (form | tile << 2)
+    int k2_variant;  // full variant code (form | tile << 2 | LDE_ROW_HALF)
     bool split;      // LDE_SPLIT: s2inv (k_lde_rowinv) runs before s2 (k_lde_expand)
     LdeArgs s2inv;
     u32 inv_logNT, inv_blocks;  // s2inv.logNT == inv_logNT
     size_t inv_lds;
     u32 col_lt;      // log2 tile of the column kernels (K1, K3)
+    u32 row_lt;      // log2 tile of the row kernels (k_lde_rowinv, k_lde_expand)
     Step1Args s1;
     LdeArgs s2;
     Step3Args s3;
