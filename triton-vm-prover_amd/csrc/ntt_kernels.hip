@@ -181,10 +181,49 @@ struct LdsIO {
     }
 };
 
+// Workgroup barrier for LDS hand-offs only: waits for this wave's LDS
+// operations, not for its global loads (the __syncthreads fence also drains
+// the vector-memory counter, which would expose the latency of twiddle loads
+// issued ahead of the barrier).  No thread of these kernels reads global
+// memory that another thread of the block writes before the barrier.
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Stockham twiddles of one pass for the thread's work items, loaded ahead of
+// the pass: tw.t[it][r] = r4096[k * r * step] (r >= 1) for work item it.
+template <int LOGR, int NTPB, int NTILE>
+struct PassTw {
+    static constexpr int R = 1 << LOGR;
+    static constexpr int MAXIT = NTILE / (R * NTPB);
+    u64 t[MAXIT][R];
+};
+
+template <int LOGR, int NTPB, int NTILE, bool TF>
+__device__ __forceinline__ void prefetch_tw(PassTw<LOGR, NTPB, NTILE>& tw, int logL, int NT,
+                                            int logNT, int Ns, const u64* __restrict__ r4096) {
+    constexpr int R = 1 << LOGR;
+    constexpr int MAXIT = NTILE / (R * NTPB);
+    const int lp = logL - LOGR;
+    const int per = 1 << lp;
+    const int total = NT << lp;
+    const int step = 4096 / (Ns * R);
+#pragma unroll
+    for (int it = 0; it < MAXIT; it++) {
+        int w = threadIdx.x + it * NTPB;
+        if (w < total) {
+            int j = TF ? (w >> logNT) : (w & (per - 1));
+            int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; r++) tw.t[it][r] = r4096[k * r * step];
+        }
+    }
+}
+
 template <int LOGR, bool INV, int NTPB, int NTILE, bool TF, bool FROM_LDS, bool TO_LDS, class Ld,
           class St>
 __device__ __forceinline__ void fused_pass(const LdsIO& io, int logL, int NT, int logNT, int Ns,
-                                           const u64* __restrict__ r4096, const Ld& ld,
+                                           const PassTw<LOGR, NTPB, NTILE>& tw, const Ld& ld,
                                            const St& st) {
     constexpr int R = 1 << LOGR;
     constexpr int MAXIT = NTILE / (R * NTPB);
@@ -192,7 +231,6 @@ __device__ __forceinline__ void fused_pass(const LdsIO& io, int logL, int NT, in
     const int lp = logL - LOGR;
     const int per = 1 << lp;
     const int total = NT << lp;
-    const int step = 4096 / (Ns * R);
     u64 v[MAXIT][R];
 #pragma unroll
     for (int it = 0; it < MAXIT; it++) {
@@ -203,14 +241,13 @@ __device__ __forceinline__ void fused_pass(const LdsIO& io, int logL, int NT, in
             if (FROM_LDS) io.load<R>(t, j, per, v[it]);
             else ld.template load<R>(t, j, per, v[it]);
             if (Ns > 1) {
-                int k = j & (Ns - 1);
 #pragma unroll
-                for (int r = 1; r < R; r++) v[it][r] = gl::mont_mul(v[it][r], r4096[k * r * step]);
+                for (int r = 1; r < R; r++) v[it][r] = gl::mont_mul(v[it][r], tw.t[it][r]);
             }
             dft_reg<LOGR, INV>(v[it]);
         }
     }
-    if (FROM_LDS && TO_LDS) __syncthreads();  // in place: every read of the pass before any write
+    if (FROM_LDS && TO_LDS) lds_barrier();  // in place: every read of the pass before any write
 #pragma unroll
     for (int it = 0; it < MAXIT; it++) {
         int w = threadIdx.x + it * NTPB;
@@ -226,17 +263,33 @@ __device__ __forceinline__ void fused_pass(const LdsIO& io, int logL, int NT, in
 // Straight-line pass sequence for L = 2^(4Q + REM): Q radix-16 passes, then a
 // radix-2^REM pass (a radix-1 copy when L = 1).  Every radix, stride and
 // first/last role is a compile-time constant, which keeps register pressure
-// at one pass's worth.
-template <int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, class Ld, class St, size_t... P>
+// at one pass's worth.  The twiddles of pass P + 1 are loaded before the
+// barrier that ends pass P, so their latency overlaps the barrier wait.
+template <int P, int NP, int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, class Ld,
+          class St>
+__device__ __forceinline__ void fused_chain(const LdsIO& io, int NT, int logNT,
+                                            const u64* __restrict__ r4096, const Ld& ld,
+                                            const St& st,
+                                            const PassTw<(P < Q ? 4 : REM), NTPB, NTILE>& tw) {
+    constexpr int logL = 4 * Q + REM;
+    fused_pass<(P < Q ? 4 : REM), INV, NTPB, NTILE, TF, (P > 0), (P + 1 < NP)>(
+        io, logL, NT, logNT, 1 << (4 * P), tw, ld, st);
+    if constexpr (P + 1 < NP) {
+        PassTw<(P + 1 < Q ? 4 : REM), NTPB, NTILE> next;
+        prefetch_tw<(P + 1 < Q ? 4 : REM), NTPB, NTILE, TF>(next, logL, NT, logNT, 1 << (4 * (P + 1)),
+                                                           r4096);
+        lds_barrier();
+        fused_chain<P + 1, NP, Q, REM, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st, next);
+    }
+}
+
+template <int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, class Ld, class St>
 __device__ __forceinline__ void fused_passes(const LdsIO& io, int NT, int logNT,
                                              const u64* __restrict__ r4096, const Ld& ld,
-                                             const St& st, std::index_sequence<P...>) {
-    constexpr int NP = (int)sizeof...(P);
-    constexpr int logL = 4 * Q + REM;
-    ((fused_pass<((int)P < Q ? 4 : REM), INV, NTPB, NTILE, TF, (P > 0), ((int)P + 1 < NP)>(
-          io, logL, NT, logNT, 1 << (4 * P), r4096, ld, st),
-      ((int)P + 1 < NP ? __syncthreads() : (void)0)),
-     ...);
+                                             const St& st) {
+    constexpr int NP = Q + ((REM != 0 || Q == 0) ? 1 : 0);
+    PassTw<(0 < Q ? 4 : REM), NTPB, NTILE> none;  // the first pass has no twiddles
+    fused_chain<0, NP, Q, REM, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st, none);
 }
 
 // LDS row pitch of a transform for the fused DFT: L + L/16 + pad, with pad
@@ -256,9 +309,7 @@ __device__ __forceinline__ void fused_dft(u64* buf, int logL, int NT, int logNT,
                                           const u64* __restrict__ r4096, const Ld& ld,
                                           const St& st) {
     const LdsIO io{buf, fused_pitch(1 << logL, NT, TF)};
-#define TVH_FUSED(q, r)                                                                      \
-    fused_passes<q, r, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st,                 \
-                                             std::make_index_sequence<(q) + ((r) != 0 || (q) == 0)>{})
+#define TVH_FUSED(q, r) fused_passes<q, r, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st)
     switch (logL) {
         case 0: TVH_FUSED(0, 0); break;
         case 1: TVH_FUSED(0, 1); break;
@@ -589,14 +640,25 @@ struct ExpandLoad {
         u64 c = s >> a, k1 = s & ((1ull << a) - 1);
         const u64* p = A.src + c * A.src_col_stride + k1 * A.src_row_stride;
         const u64* pb = A.pb + ((u64)sl << b);
+        // every load first, with no branch in between, so that they are in
+        // flight together
+        u64 f[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            int k2 = j + r * per;
-            u64 x = p[k2];
-            u64 rr = k1 + ((u64)k2 << a);
-            if (rr < A.rs.k) x = gl::add(x, rand_term(A, c, sl, rr));
-            v[r] = gl::mont_mul(x, pb[k2]);
+            v[r] = p[j + r * per];
+            f[r] = pb[j + r * per];
         }
+        // randomizer terms: element k2 of row k1 is trace row k1 + k2 * N1,
+        // which carries one when below rand_len (possible only for k1 < rand_len)
+        if (k1 < A.rs.k) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                u64 rr = k1 + ((u64)(j + r * per) << a);
+                if (rr < A.rs.k) v[r] = gl::add(v[r], rand_term(A, c, sl, rr));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) v[r] = gl::mont_mul(v[r], f[r]);
     }
 };
 // Output u1 of coset sl, row k1, times N^-1 beta_sl^k1 w_N^(k1*u1), to
