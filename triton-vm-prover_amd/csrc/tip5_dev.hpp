@@ -66,8 +66,7 @@ constexpr RoundConstantsM mont_round_constants() {
         t.v[i] = (u64)(((unsigned __int128)kRoundConstants[i] * gl::EPS) % gl::P);
     return t;
 }
-// The round constants in Montgomery form (c * 2^64 mod p, 2^64 = 2^32 - 1 mod p).
-__constant__ RoundConstantsM kRcMont = mont_round_constants();
+// (the round constants in Montgomery form: c * 2^64 mod p, 2^64 = 2^32 - 1 mod p)
 
 // 1 in Montgomery form (2^64 mod p)
 constexpr u64 kOneM = gl::EPS;
@@ -104,11 +103,29 @@ __device__ __forceinline__ u64 pow7_m(u64 x) {
     return gl::mont_mul(x3, x4);
 }
 
-// integer-valued double in [0, 2^52) -> u64
-__device__ __forceinline__ u64 f2u52(double a) {
-    double b = a + 4503599627370496.0;  // 2^52
-    return (u64)__double_as_longlong(b) & 0x000FFFFFFFFFFFFFull;
+// integer-valued double a >= 0 and bias = 2^52 + c with a + c < 2^52 -> a + c
+__device__ __forceinline__ u64 f2u52(double a, double bias) {
+    return (u64)__double_as_longlong(a + bias) & 0x000FFFFFFFFFFFFFull;
 }
+
+// Round constants as f64 biases for the MDS recombination: adding the 32-bit
+// halves c0, c1 of the Montgomery-form constant to the exact MDS outputs
+// (L + c0) + (H + c1) * 2^32 adds the constant itself; 2^52 + c is exact in f64
+// and L + c0, H + c1 stay below 2^52.
+struct RcBias {
+    double v[ROUNDS * STATE * 2];
+};
+constexpr RcBias rc_bias_table() {
+    RcBias t{};
+    const RoundConstantsM m = mont_round_constants();
+    for (int i = 0; i < ROUNDS * STATE; i++) {
+        u64 c = m.v[i];
+        t.v[2 * i] = 4503599627370496.0 + (double)(u32)c;
+        t.v[2 * i + 1] = 4503599627370496.0 + (double)(u32)(c >> 32);
+    }
+    return t;
+}
+__constant__ RcBias kRcBias = rc_bias_table();
 
 // Circulant MDS y_r = sum_c M[(r - c) & 15] s_c on the two 32-bit halves of the
 // state, each via x^16 - 1 = (x^8 - 1)(x^8 + 1): a cyclic 8-term product of
@@ -143,7 +160,8 @@ __device__ __forceinline__ void mds_half(const double x[STATE], double y[STATE])
     }
 }
 
-__device__ __forceinline__ void mds(u64 s[STATE]) {
+// MDS followed by the round constants of round `round` (any u64 in, < 2^64 out).
+__device__ __forceinline__ void mds_rc(u64 s[STATE], int round) {
     double lo[STATE], hi[STATE], ylo[STATE], yhi[STATE];
 #pragma unroll
     for (int c = 0; c < STATE; c++) {
@@ -152,9 +170,10 @@ __device__ __forceinline__ void mds(u64 s[STATE]) {
     }
     mds_half(lo, ylo);
     mds_half(hi, yhi);
+    const double* bias = kRcBias.v + round * STATE * 2;
 #pragma unroll
-    for (int r = 0; r < STATE; r++) {
-        s[r] = gl::lin52(f2u52(ylo[r]), f2u52(yhi[r]));  // L + H * 2^32
+    for (int r = 0; r < STATE; r++) {  // (L + c0) + (H + c1) * 2^32
+        s[r] = gl::lin52(f2u52(ylo[r], bias[2 * r]), f2u52(yhi[r], bias[2 * r + 1]));
     }
 }
 
@@ -166,9 +185,7 @@ __device__ __forceinline__ void permutation_m(u64 s[STATE], const uint8_t* lut) 
         for (int i = 0; i < 4; i++) s[i] = split_and_lookup_m(s[i], lut);
 #pragma unroll
         for (int i = 4; i < STATE; i++) s[i] = pow7_m(s[i]);
-        mds(s);
-#pragma unroll
-        for (int i = 0; i < STATE; i++) s[i] = gl::add_lazy(s[i], kRcMont.v[r * STATE + i]);
+        mds_rc(s, r);
     }
 }
 
