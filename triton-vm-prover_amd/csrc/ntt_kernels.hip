@@ -379,12 +379,20 @@ struct Step1Store {
         u64 n2 = s & ((1ull << b) - 1);
         u64* p = dst + (s >> b) * dstride + n2;
         int k = j & (Ns - 1), base = (j - k) * R + k;
+        // table entries first (index 0 holds 1, so n2 = 0 or k1 = 0 needs no
+        // branch), then the products: the loads are in flight together
+        const u32 sh = kMaxLogN - logN;
+        u64 tl[R], th[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            u32 y = (u32)(n2 * (u32)(base + r * Ns)) << sh;
+            tl[r] = rt.lo[INV][y & 4095];
+            th[r] = rt.hi[INV][y >> 12];
+        }
 #pragma unroll
         for (int r = 0; r < R; r++) {
             u32 k1 = (u32)(base + r * Ns);
-            u64 x = v[r];
-            if (n2 && k1) x = gl::mont_mul(x, twiddle(rt, INV, (u32)(n2 * k1), logN));
-            p[((u64)E * k1) << b] = x;
+            p[((u64)E * k1) << b] = gl::mont_mul(v[r], gl::mont_mul(tl[r], th[r]));
         }
     }
 };
@@ -619,19 +627,21 @@ __global__ void __launch_bounds__(TPB) k_lde_rowinv_multi(const LdeArgs* __restr
     lde_rowinv_body(A, blockIdx.x - start[seg], rt);
 }
 
-// Coset transforms T = t*E + sl of row t (coefficients at src rows, written
-// by k_lde_rowinv): element k2 = (C_k2 + randomizer term) * beta_sl^(N1*k2).
-// Cosets are the fastest index, so that with the transform-fastest thread
-// mapping the E cosets of one output position are written by adjacent lanes
-// (out[E*u1 + sl] contiguous).
+// Coset transforms T = t*Eb + (sl - sl0) of row t (coefficients at src rows,
+// written by k_lde_rowinv), for the Eb = 2^logEb cosets sl0.. of this block:
+// element k2 = (C_k2 + randomizer term) * beta_sl^(N1*k2).  Cosets are the
+// fastest index, so that with the transform-fastest thread mapping the cosets
+// of one output position are written by adjacent lanes (out[E*u1 + sl]
+// contiguous).
 struct ExpandLoad {
     const LdeArgs& A;
     u64 s0, nrows;
+    u32 sl0;
     template <int R>
     __device__ __forceinline__ void load(int T, int j, int per, u64 (&v)[R]) const {
         const u32 a = A.a, b = A.logN - A.a;
-        u32 sl = (u32)T & (A.E - 1);
-        u64 s = s0 + ((u32)T >> A.logE);
+        u32 sl = sl0 + ((u32)T & ((1u << A.logEb) - 1));
+        u64 s = s0 + ((u32)T >> A.logEb);
         if (s >= nrows) {
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = 0;
@@ -668,11 +678,12 @@ struct ExpandStore {
     const LdeArgs& A;
     Roots rt;
     u64 s0, nrows;
+    u32 sl0;
     template <int R>
     __device__ __forceinline__ void store(int T, int j, int Ns, const u64 (&v)[R]) const {
         const u32 a = A.a, b = A.logN - A.a, E = A.E;
-        u32 sl = (u32)T & (E - 1);
-        u64 s = s0 + ((u32)T >> A.logE);
+        u32 sl = sl0 + ((u32)T & ((1u << A.logEb) - 1));
+        u64 s = s0 + ((u32)T >> A.logEb);
         if (s >= nrows) return;
         u64 c = s >> a, k1 = s & ((1ull << a) - 1);
         u64* dst = A.out + c * A.M + (((u64)E * k1) << b) + sl;
@@ -697,18 +708,21 @@ struct ExpandStore {
     }
 };
 
-// Split form, second kernel: for NT rows per block (NT * E * N2 = 2^LT), the E
-// coset transforms straight from the coefficient rows to the codewords.
+// Split form, second kernel: the coset transforms straight from the
+// coefficient rows to the codewords, NT rows x Eb cosets per block
+// (NT * Eb * N2 = 2^LT); with Eb < E the E / Eb blocks of a row are adjacent.
 template <int LT = LOG_TILE>
 __device__ __forceinline__ void lde_expand_body(const LdeArgs& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
     const u32 b = A.logN - A.a;
     const u64 nrows = A.cols << A.a;
-    const u64 s0 = blk << A.logNT;
-    const ExpandLoad ld{A, s0, nrows};
-    const ExpandStore st{A, rt, s0, nrows};
+    const u32 h = A.logE - A.logEb;
+    const u64 s0 = (blk >> h) << A.logNT;
+    const u32 sl0 = ((u32)blk & ((1u << h) - 1)) << A.logEb;
+    const ExpandLoad ld{A, s0, nrows, sl0};
+    const ExpandStore st{A, rt, s0, nrows, sl0};
     fused_dft<false, (1 << (LT - 4)), (1 << LT), true>(
-        lds, b, (int)(A.E << A.logNT), (int)(A.logE + A.logNT), rt.r4096[0], ld, st);
+        lds, b, (int)(1u << (A.logEb + A.logNT)), (int)(A.logEb + A.logNT), rt.r4096[0], ld, st);
 }
 
 template <int LT>
@@ -860,12 +874,12 @@ int set_lde_variant(u32 logN, u32 logE, int v) {
     return TVH_OK;
 }
 
-static u32 choose_split(u32 logN, u32 E, u32 row_lt = LOG_TILE) {
-    // N2 (the contiguous row step) as long as one row tile allows for all E
-    // cosets (E * N2 <= 2^row_lt, N2 <= 4096); N1 = N / N2 takes the strided
-    // step, whose HBM segments are (column tile) / N1 elements wide.
+static u32 choose_split(u32 logN, u32 E) {
+    // N2 (the contiguous row step) as long as one full tile allows for all E
+    // cosets (E * N2 <= TILE, N2 <= 4096); N1 = N / N2 takes the strided step,
+    // whose HBM segments are (column tile) / N1 elements wide.
     u32 logE = log2u(E);
-    u32 bmax = row_lt - logE;
+    u32 bmax = LOG_TILE - logE;
     if (bmax > 12) bmax = 12;
     u32 b = logN < bmax ? logN : bmax;
     return logN - b;
@@ -966,7 +980,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
     if (logN > 23) return set_error(TVH_ERR_UNSUPPORTED, "trace_len > 2^23");
     DeviceState* ds = current_device();
     if (!ds) return TVH_ERR_DEVICE;
-    u32 a = choose_split(logN, E, P->row_lt);
+    u32 a = choose_split(logN, E);
     u32 b = logN - a;
     if (a > 12) return set_error(TVH_ERR_UNSUPPORTED, "split");
     ConstVal cv;
@@ -1005,18 +1019,27 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
             A.delta_n[s] = gl::mul(d, nmod);
             ws = gl::mul(ws, wE);
         }
-        u32 logNT = P->row_lt - (b + logE);  // NT * E * N2 = 2^row_lt
+        // NT * Eb * N2 = 2^row_lt: all E cosets of NT rows per block, or
+        // (when E * N2 exceeds the row tile) Eb = 2^row_lt / N2 cosets of one row
+        u32 logEb = logE, logNT;
         if (b < 5) logNT = P->row_lt - 5 - logE;
+        else if (b + logE <= P->row_lt) logNT = P->row_lt - (b + logE);
+        else {
+            logEb = P->row_lt - b;
+            logNT = 0;
+        }
         u64 nrows = cols << a;
         if ((1ull << logNT) > nrows) logNT = log2u(nrows);
         A.logNT = logNT;
+        A.logEb = logEb;
         A.src = P->k1 ? out : trace;
         A.src_col_stride = P->k1 ? m : n;
         A.src_row_stride = P->k1 ? ((u64)E << b) : 0;
-        P->s2_blocks = (u32)((nrows + (1ull << logNT) - 1) >> logNT);
+        P->s2_blocks = (u32)(((nrows + (1ull << logNT) - 1) >> logNT) << (logE - logEb));
         P->s2_lds = lds_bytes(b, logNT + logE);
         if ((P->k2_variant & 3) == LDE_SPLIT)  // k_lde_expand: transform-fastest pitch
-            P->s2_lds = (size_t)fused_pitch(1 << b, (int)(E << logNT), true) * (E << logNT) * sizeof(u64);
+            P->s2_lds = (size_t)fused_pitch(1 << b, 1 << (logEb + logNT), true) *
+                        (1u << (logEb + logNT)) * sizeof(u64);
         P->split = (P->k2_variant & 3) == LDE_SPLIT;
         if (P->split) {  // inverse rows first (TILE / N2 rows per block), then expansion from out
             u32 li = tile_log_nt(b, P->row_lt);

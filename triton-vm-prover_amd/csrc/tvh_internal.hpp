@@ -110,14 +110,15 @@ struct LdeArgs {
     u64 toff;          // o
     RandSpec rs;
     u32 logNT;
+    u32 logEb;         // expansion: 2^logEb of the E cosets per block (logEb < logE splits a row)
 };
 // LDE kernel variant: bits 0-1 the form of the fused row step (LDE_WIDE: one
 // kernel, all E cosets of a row in one 1024-thread LDS tile; LDE_SPLIT: the
 // inverse row DFTs as a kernel of their own, then the expansion); bits 2-3 the
 // column kernels' tile, 2^(14 - code) elements with 16 per thread; bit 4
 // (LDE_SPLIT only) halves the row kernels' tile to 2^13 elements (512 threads,
-// two blocks per CU), which also halves the row length N2 when E * N2 would
-// not fit.
+// two blocks per CU); where the E cosets of a row do not fit such a tile, the
+// expansion splits them over several blocks (each re-reads the row).
 enum LdeVariant { LDE_AUTO = 0, LDE_WIDE = 1, LDE_SPLIT = 2, LDE_ROW_HALF = 16 };
 int lde_variant(u32 logN, u32 logE);            // the tuned choice (never LDE_AUTO)
 bool lde_variant_valid(int v);
