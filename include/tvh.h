@@ -140,8 +140,9 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t t
  * instead of 2^14, with the row length halved where E * row length would not
  * fit.  The variant is chosen per
  * (log2 trace_len, log2 expansion); TVH_LDE_AUTO restores the default
- * (TVH_LDE_SPLIT with tile 2 from 2^14 rows on, TVH_LDE_WIDE with tile 2
- * below).  Batch plans use each table's form with tile 2.
+ * (TVH_LDE_SPLIT | TVH_LDE_ROW_HALF with tile 2 from 2^12 rows on, tile 1 from
+ * 2^20; TVH_LDE_WIDE with tile 2 below).  Batch plans use each table's form
+ * with column tile 2 and 2^13-element row tiles.
  * No reference counterpart (the reference has one fixed kernel schedule per
  * size, src/gpu/kernels/ntt_kernel.cu). */
 enum { TVH_LDE_AUTO = 0, TVH_LDE_WIDE = 1, TVH_LDE_SPLIT = 2, TVH_LDE_ROW_HALF = 16 };

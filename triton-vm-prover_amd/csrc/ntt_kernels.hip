@@ -619,12 +619,13 @@ __global__ void __launch_bounds__(1 << (LT - 4)) k_lde_rowinv(LdeArgs A, Roots r
     lde_rowinv_body<LT>(A, blockIdx.x, rt);
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_rowinv_multi(const LdeArgs* __restrict__ args,
-                                                          const u32* __restrict__ start, u32 nseg,
-                                                          Roots rt) {
+template <int LT>
+__global__ void __launch_bounds__(1 << (LT - 4)) k_lde_rowinv_multi(const LdeArgs* __restrict__ args,
+                                                                    const u32* __restrict__ start,
+                                                                    u32 nseg, Roots rt) {
     u32 seg = find_segment(start, nseg, blockIdx.x);
     const LdeArgs& A = args[seg];
-    lde_rowinv_body(A, blockIdx.x - start[seg], rt);
+    lde_rowinv_body<LT>(A, blockIdx.x - start[seg], rt);
 }
 
 // Coset transforms T = t*Eb + (sl - sl0) of row t (coefficients at src rows,
@@ -730,12 +731,13 @@ __global__ void __launch_bounds__(1 << (LT - 4)) k_lde_expand(LdeArgs A, Roots r
     lde_expand_body<LT>(A, blockIdx.x, rt);
 }
 
-__global__ void __launch_bounds__(TPB) k_lde_expand_multi(const LdeArgs* __restrict__ args,
-                                                          const u32* __restrict__ start, u32 nseg,
-                                                          Roots rt) {
+template <int LT>
+__global__ void __launch_bounds__(1 << (LT - 4)) k_lde_expand_multi(const LdeArgs* __restrict__ args,
+                                                                    const u32* __restrict__ start,
+                                                                    u32 nseg, Roots rt) {
     u32 seg = find_segment(start, nseg, blockIdx.x);
     const LdeArgs& A = args[seg];
-    lde_expand_body(A, blockIdx.x - start[seg], rt);
+    lde_expand_body<LT>(A, blockIdx.x - start[seg], rt);
 }
 
 __global__ void __launch_bounds__(TPB) k_lde_step2_multi(const LdeArgs* __restrict__ args,
@@ -856,9 +858,11 @@ static std::atomic<int> g_variant[kMaxLogN + 1][5];
 int lde_variant(u32 logN, u32 logE) {
     int v = (logN <= kMaxLogN && logE <= 4) ? g_variant[logN][logE].load() : 0;
     if (v) return v;
-    // default (measured on MI355X at widths 156..379): the split form from 2^14
-    // rows on, the single fused kernel below; column tiles of 2^12
-    return (logN >= 14 ? LDE_SPLIT : LDE_WIDE) | (2 << 2);
+    // default (measured on MI355X at widths 156..379, 2^12..2^20 rows): the split
+    // form with 2^13-element row tiles from 2^12 rows on (column tiles 2^12, 2^13
+    // from 2^20 rows), the single fused kernel below
+    if (logN >= 20) return LDE_SPLIT | (1 << 2) | LDE_ROW_HALF;
+    return logN >= 12 ? (LDE_SPLIT | (2 << 2) | LDE_ROW_HALF) : (LDE_WIDE | (2 << 2));
 }
 
 bool lde_variant_valid(int v) {
@@ -966,7 +970,8 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
     // multi-table launches run the column kernels with the 2^12 tile and the
     // row kernels with the full tile
     P->col_lt = batch ? kBatchColLt : LOG_TILE - ((P->k2_variant >> 2) & 3);
-    P->row_lt = (!batch && (P->k2_variant & LDE_ROW_HALF)) ? LOG_TILE - 1 : LOG_TILE;
+    P->row_lt = batch ? kBatchRowLt : (P->k2_variant & LDE_ROW_HALF) ? LOG_TILE - 1 : LOG_TILE;
+    if ((P->k2_variant & 3) != LDE_SPLIT) P->row_lt = LOG_TILE;  // the wide kernel's tile
     P->s1_blocks = P->s2_blocks = P->s3_blocks = 0;
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
@@ -1124,14 +1129,14 @@ int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg
     if (blocks == 0) return TVH_OK;
     DeviceState* ds = current_device();
     if (!ds) return TVH_ERR_DEVICE;
-    const dim3 ctpb(1u << (kBatchColLt - 4));
+    const dim3 ctpb(1u << (kBatchColLt - 4)), rtpb(1u << (kBatchRowLt - 4));
     switch (phase) {
         case LDE_PHASE_STEP1:
             hipLaunchKernelGGL((k_step1_multi<true, kBatchColLt>), dim3(blocks), ctpb, lds, st,
                                (const Step1Args*)d_args, d_start, nseg, ds->roots);
             break;
         case LDE_PHASE_ROWINV:
-            hipLaunchKernelGGL(k_lde_rowinv_multi, dim3(blocks), dim3(TPB), lds, st,
+            hipLaunchKernelGGL(k_lde_rowinv_multi<kBatchRowLt>, dim3(blocks), rtpb, lds, st,
                                (const LdeArgs*)d_args, d_start, nseg, ds->roots);
             break;
         case LDE_PHASE_STEP2:
@@ -1139,7 +1144,7 @@ int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg
                                (const LdeArgs*)d_args, d_start, nseg, ds->roots);
             break;
         case LDE_PHASE_EXPAND:
-            hipLaunchKernelGGL(k_lde_expand_multi, dim3(blocks), dim3(TPB), lds, st,
+            hipLaunchKernelGGL(k_lde_expand_multi<kBatchRowLt>, dim3(blocks), rtpb, lds, st,
                                (const LdeArgs*)d_args, d_start, nseg, ds->roots);
             break;
         default:
@@ -1218,8 +1223,8 @@ int ntt_set_attributes() {
                          (const void*)k_lde_step2_multi,
                          (const void*)k_lde_step3_multi<kBatchColLt>, (const void*)k_lde_rowinv<LOG_TILE>,
                          (const void*)k_lde_expand<LOG_TILE>, (const void*)k_lde_rowinv<LOG_TILE - 1>,
-                         (const void*)k_lde_expand<LOG_TILE - 1>, (const void*)k_lde_rowinv_multi,
-                         (const void*)k_lde_expand_multi,
+                         (const void*)k_lde_expand<LOG_TILE - 1>, (const void*)k_lde_rowinv_multi<kBatchRowLt>,
+                         (const void*)k_lde_expand_multi<kBatchRowLt>,
 };
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)

@@ -139,7 +139,8 @@ struct LdePlan {
     u32 s1_blocks, s2_blocks, s3_blocks;
     size_t s1_lds, s2_lds, s3_lds;
 };
-// batch = true plans the column kernels with the multi-table tile (kBatchColLt).
+// batch = true plans the column and row kernels with the multi-table tiles
+// (kBatchColLt, kBatchRowLt; the wide form keeps its full tile).
 int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
                 u64 m, bool truncate, u64* out, LdePlan* plan, bool batch);
 // Phases of the multi-table LDE, in launch order.
@@ -152,6 +153,10 @@ enum LdePhase {
     LDE_PHASES = 5
 };
 constexpr u32 kBatchColLt = 12;  // column-kernel tile of multi-table launches
+#ifndef TVH_BATCH_ROW_LT
+#define TVH_BATCH_ROW_LT 13
+#endif
+constexpr u32 kBatchRowLt = TVH_BATCH_ROW_LT;  // row-kernel tile of multi-table launches
 int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg, u32 blocks,
                      size_t lds, hipStream_t st);
 int launch_fill_uniform(u64* d, u64 count, u64 seed, hipStream_t st);
