@@ -203,7 +203,8 @@ int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant) {
     return set_lde_variant(log_n, log_e, variant);
 }
 
-static const int kLdeVariants[] = {1, 2, 5, 6, 9, 10, 18, 22, 26};  // form | tile << 2 | half
+// form | column tile << 2 | LDE_ROW_HALF (16) or LDE_INV_HALF (32)
+static const int kLdeVariants[] = {1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42};
 static constexpr int kNumLdeVariants = sizeof(kLdeVariants) / sizeof(kLdeVariants[0]);
 
 int tvh_lde_variant_count(void) { return kNumLdeVariants; }

@@ -628,21 +628,19 @@ __global__ void __launch_bounds__(1 << (LT - 4)) k_lde_rowinv_multi(const LdeArg
     lde_rowinv_body<LT>(A, blockIdx.x - start[seg], rt);
 }
 
-// Coset transforms T = t*Eb + (sl - sl0) of row t (coefficients at src rows,
-// written by k_lde_rowinv), for the Eb = 2^logEb cosets sl0.. of this block:
-// element k2 = (C_k2 + randomizer term) * beta_sl^(N1*k2).  Cosets are the
-// fastest index, so that with the transform-fastest thread mapping the cosets
-// of one output position are written by adjacent lanes (out[E*u1 + sl]
-// contiguous).
+// Coset transforms T = t*E + sl of row t (coefficients at src rows, written
+// by k_lde_rowinv): element k2 = (C_k2 + randomizer term) * beta_sl^(N1*k2).
+// Cosets are the fastest index, so that with the transform-fastest thread
+// mapping the E cosets of one output position are written by adjacent lanes
+// (out[E*u1 + sl] contiguous).
 struct ExpandLoad {
     const LdeArgs& A;
     u64 s0, nrows;
-    u32 sl0;
     template <int R>
     __device__ __forceinline__ void load(int T, int j, int per, u64 (&v)[R]) const {
         const u32 a = A.a, b = A.logN - A.a;
-        u32 sl = sl0 + ((u32)T & ((1u << A.logEb) - 1));
-        u64 s = s0 + ((u32)T >> A.logEb);
+        u32 sl = (u32)T & (A.E - 1);
+        u64 s = s0 + ((u32)T >> A.logE);
         if (s >= nrows) {
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = 0;
@@ -679,12 +677,11 @@ struct ExpandStore {
     const LdeArgs& A;
     Roots rt;
     u64 s0, nrows;
-    u32 sl0;
     template <int R>
     __device__ __forceinline__ void store(int T, int j, int Ns, const u64 (&v)[R]) const {
         const u32 a = A.a, b = A.logN - A.a, E = A.E;
-        u32 sl = sl0 + ((u32)T & ((1u << A.logEb) - 1));
-        u64 s = s0 + ((u32)T >> A.logEb);
+        u32 sl = (u32)T & (E - 1);
+        u64 s = s0 + ((u32)T >> A.logE);
         if (s >= nrows) return;
         u64 c = s >> a, k1 = s & ((1ull << a) - 1);
         u64* dst = A.out + c * A.M + (((u64)E * k1) << b) + sl;
@@ -709,21 +706,21 @@ struct ExpandStore {
     }
 };
 
-// Split form, second kernel: the coset transforms straight from the
-// coefficient rows to the codewords, NT rows x Eb cosets per block
-// (NT * Eb * N2 = 2^LT); with Eb < E the E / Eb blocks of a row are adjacent.
+// Split form, second kernel: for NT rows per block (NT * E * N2 = 2^LT), the E
+// coset transforms straight from the coefficient rows to the codewords.  The
+// outputs overwrite the coefficient row in place, so every coset of a row
+// must be computed by the block that reads the row (its reads all precede its
+// first barrier, its writes follow its last one).
 template <int LT = LOG_TILE>
 __device__ __forceinline__ void lde_expand_body(const LdeArgs& A, u64 blk, const Roots& rt) {
     extern __shared__ u64 lds[];
     const u32 b = A.logN - A.a;
     const u64 nrows = A.cols << A.a;
-    const u32 h = A.logE - A.logEb;
-    const u64 s0 = (blk >> h) << A.logNT;
-    const u32 sl0 = ((u32)blk & ((1u << h) - 1)) << A.logEb;
-    const ExpandLoad ld{A, s0, nrows, sl0};
-    const ExpandStore st{A, rt, s0, nrows, sl0};
+    const u64 s0 = blk << A.logNT;
+    const ExpandLoad ld{A, s0, nrows};
+    const ExpandStore st{A, rt, s0, nrows};
     fused_dft<false, (1 << (LT - 4)), (1 << LT), true>(
-        lds, b, (int)(1u << (A.logEb + A.logNT)), (int)(A.logEb + A.logNT), rt.r4096[0], ld, st);
+        lds, b, (int)(A.E << A.logNT), (int)(A.logE + A.logNT), rt.r4096[0], ld, st);
 }
 
 template <int LT>
@@ -866,9 +863,9 @@ int lde_variant(u32 logN, u32 logE) {
 }
 
 bool lde_variant_valid(int v) {
-    int form = v & 3, tile = (v >> 2) & 3, half = v >> 4;
-    return v == LDE_AUTO ||
-           ((form == LDE_WIDE || form == LDE_SPLIT) && tile <= 2 && half <= (form == LDE_SPLIT));
+    int form = v & 3, tile = (v >> 2) & 3, half = (v >> 4) & 1, ihalf = v >> 5;
+    return v == LDE_AUTO || ((form == LDE_WIDE || form == LDE_SPLIT) && tile <= 2 &&
+                             half + ihalf <= (form == LDE_SPLIT));
 }
 
 int set_lde_variant(u32 logN, u32 logE, int v) {
@@ -878,12 +875,14 @@ int set_lde_variant(u32 logN, u32 logE, int v) {
     return TVH_OK;
 }
 
-static u32 choose_split(u32 logN, u32 E) {
-    // N2 (the contiguous row step) as long as one full tile allows for all E
-    // cosets (E * N2 <= TILE, N2 <= 4096); N1 = N / N2 takes the strided step,
-    // whose HBM segments are (column tile) / N1 elements wide.
+static u32 choose_split(u32 logN, u32 E, u32 row_lt) {
+    // N2 (the contiguous row step) as long as one row tile holds all E cosets
+    // of a row (E * N2 <= 2^row_lt, N2 <= 4096: the expansion overwrites its
+    // input row in place, so a row's cosets cannot be split over blocks);
+    // N1 = N / N2 takes the strided step, whose HBM segments are (column
+    // tile) / N1 elements wide.
     u32 logE = log2u(E);
-    u32 bmax = LOG_TILE - logE;
+    u32 bmax = row_lt - logE;
     if (bmax > 12) bmax = 12;
     u32 b = logN < bmax ? logN : bmax;
     return logN - b;
@@ -971,6 +970,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
     // row kernels with the full tile
     P->col_lt = batch ? kBatchColLt : LOG_TILE - ((P->k2_variant >> 2) & 3);
     P->row_lt = batch ? kBatchRowLt : (P->k2_variant & LDE_ROW_HALF) ? LOG_TILE - 1 : LOG_TILE;
+    P->inv_lt = (P->row_lt < LOG_TILE || (P->k2_variant & LDE_INV_HALF)) ? LOG_TILE - 1 : LOG_TILE;
     if ((P->k2_variant & 3) != LDE_SPLIT) P->row_lt = LOG_TILE;  // the wide kernel's tile
     P->s1_blocks = P->s2_blocks = P->s3_blocks = 0;
     if (cols == 0) return TVH_OK;
@@ -985,7 +985,7 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
     if (logN > 23) return set_error(TVH_ERR_UNSUPPORTED, "trace_len > 2^23");
     DeviceState* ds = current_device();
     if (!ds) return TVH_ERR_DEVICE;
-    u32 a = choose_split(logN, E);
+    u32 a = choose_split(logN, E, P->row_lt);
     u32 b = logN - a;
     if (a > 12) return set_error(TVH_ERR_UNSUPPORTED, "split");
     ConstVal cv;
@@ -1024,30 +1024,21 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
             A.delta_n[s] = gl::mul(d, nmod);
             ws = gl::mul(ws, wE);
         }
-        // NT * Eb * N2 = 2^row_lt: all E cosets of NT rows per block, or
-        // (when E * N2 exceeds the row tile) Eb = 2^row_lt / N2 cosets of one row
-        u32 logEb = logE, logNT;
+        u32 logNT = P->row_lt - (b + logE);  // NT * E * N2 = 2^row_lt
         if (b < 5) logNT = P->row_lt - 5 - logE;
-        else if (b + logE <= P->row_lt) logNT = P->row_lt - (b + logE);
-        else {
-            logEb = P->row_lt - b;
-            logNT = 0;
-        }
         u64 nrows = cols << a;
         if ((1ull << logNT) > nrows) logNT = log2u(nrows);
         A.logNT = logNT;
-        A.logEb = logEb;
         A.src = P->k1 ? out : trace;
         A.src_col_stride = P->k1 ? m : n;
         A.src_row_stride = P->k1 ? ((u64)E << b) : 0;
-        P->s2_blocks = (u32)(((nrows + (1ull << logNT) - 1) >> logNT) << (logE - logEb));
+        P->s2_blocks = (u32)((nrows + (1ull << logNT) - 1) >> logNT);
         P->s2_lds = lds_bytes(b, logNT + logE);
         if ((P->k2_variant & 3) == LDE_SPLIT)  // k_lde_expand: transform-fastest pitch
-            P->s2_lds = (size_t)fused_pitch(1 << b, 1 << (logEb + logNT), true) *
-                        (1u << (logEb + logNT)) * sizeof(u64);
+            P->s2_lds = (size_t)fused_pitch(1 << b, (int)(E << logNT), true) * (E << logNT) * sizeof(u64);
         P->split = (P->k2_variant & 3) == LDE_SPLIT;
         if (P->split) {  // inverse rows first (TILE / N2 rows per block), then expansion from out
-            u32 li = tile_log_nt(b, P->row_lt);
+            u32 li = tile_log_nt(b, P->inv_lt);
             if ((1ull << li) > nrows) li = log2u(nrows);
             P->inv_logNT = li;
             P->inv_blocks = (u32)((nrows + (1ull << li) - 1) >> li);
@@ -1091,12 +1082,12 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
         if ((rc = check_launch())) return rc;
     }
     if (P.split) {
-        const dim3 rtpb(1u << (P.row_lt - 4));
-        if (P.row_lt == LOG_TILE)
-            hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE>, dim3(P.inv_blocks), rtpb, P.inv_lds, st,
+        const dim3 rtpb(1u << (P.row_lt - 4)), itpb(1u << (P.inv_lt - 4));
+        if (P.inv_lt == LOG_TILE)
+            hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE>, dim3(P.inv_blocks), itpb, P.inv_lds, st,
                                P.s2inv, ds->roots);
         else
-            hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE - 1>, dim3(P.inv_blocks), rtpb, P.inv_lds, st,
+            hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE - 1>, dim3(P.inv_blocks), itpb, P.inv_lds, st,
                                P.s2inv, ds->roots);
         if ((rc = check_launch())) return rc;
         if (P.row_lt == LOG_TILE)

@@ -110,16 +110,16 @@ struct LdeArgs {
     u64 toff;          // o
     RandSpec rs;
     u32 logNT;
-    u32 logEb;         // expansion: 2^logEb of the E cosets per block (logEb < logE splits a row)
 };
 // LDE kernel variant: bits 0-1 the form of the fused row step (LDE_WIDE: one
 // kernel, all E cosets of a row in one 1024-thread LDS tile; LDE_SPLIT: the
 // inverse row DFTs as a kernel of their own, then the expansion); bits 2-3 the
 // column kernels' tile, 2^(14 - code) elements with 16 per thread; bit 4
 // (LDE_SPLIT only) halves the row kernels' tile to 2^13 elements (512 threads,
-// two blocks per CU); where the E cosets of a row do not fit such a tile, the
-// expansion splits them over several blocks (each re-reads the row).
-enum LdeVariant { LDE_AUTO = 0, LDE_WIDE = 1, LDE_SPLIT = 2, LDE_ROW_HALF = 16 };
+// two blocks per CU), which also halves the row length N2 when E * N2 would
+// not fit (all cosets of a row stay in one block); bit 5 (LDE_SPLIT only, not
+// with bit 4) gives only the inverse row kernel the 2^13 tile.
+enum LdeVariant { LDE_AUTO = 0, LDE_WIDE = 1, LDE_SPLIT = 2, LDE_ROW_HALF = 16, LDE_INV_HALF = 32 };
 int lde_variant(u32 logN, u32 logE);            // the tuned choice (never LDE_AUTO)
 bool lde_variant_valid(int v);
 int set_lde_variant(u32 logN, u32 logE, int v);  // v = LDE_AUTO restores the default
@@ -132,7 +132,8 @@ struct LdePlan {
     u32 inv_logNT, inv_blocks;  // s2inv.logNT == inv_logNT
     size_t inv_lds;
     u32 col_lt;      // log2 tile of the column kernels (K1, K3)
-    u32 row_lt;      // log2 tile of the row kernels (k_lde_rowinv, k_lde_expand)
+    u32 row_lt;      // log2 tile of the row kernels (k_lde_expand; the split's bound)
+    u32 inv_lt;      // log2 tile of k_lde_rowinv
     Step1Args s1;
     LdeArgs s2;
     Step3Args s3;
