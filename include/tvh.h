@@ -141,9 +141,10 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t t
  * fit; TVH_LDE_INV_HALF (with TVH_LDE_SPLIT only, not with TVH_LDE_ROW_HALF)
  * gives only the inverse row kernel the 2^13 tile.  The variant is chosen per
  * (log2 trace_len, log2 expansion); TVH_LDE_AUTO restores the default
- * (TVH_LDE_SPLIT | TVH_LDE_ROW_HALF with tile 2 from 2^12 rows on, tile 1 from
- * 2^20; TVH_LDE_WIDE with tile 2 below).  Batch plans use each table's form
- * with column tile 2 and 2^13-element row tiles.
+ * (TVH_LDE_SPLIT | TVH_LDE_ROW_HALF with tile 2 from 2^13 rows on, tile 1 from
+ * 2^18; TVH_LDE_SPLIT | TVH_LDE_INV_HALF with tile 1 at 2^12; TVH_LDE_WIDE with
+ * tile 2 below).  Batch plans use each table's form with column tile 2 and
+ * 2^13-element row tiles.
  * No reference counterpart (the reference has one fixed kernel schedule per
  * size, src/gpu/kernels/ntt_kernel.cu). */
 enum {

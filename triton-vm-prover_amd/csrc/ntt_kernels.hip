@@ -856,10 +856,12 @@ int lde_variant(u32 logN, u32 logE) {
     int v = (logN <= kMaxLogN && logE <= 4) ? g_variant[logN][logE].load() : 0;
     if (v) return v;
     // default (measured on MI355X at widths 156..379, 2^12..2^20 rows): the split
-    // form with 2^13-element row tiles from 2^12 rows on (column tiles 2^12, 2^13
-    // from 2^20 rows), the single fused kernel below
-    if (logN >= 20) return LDE_SPLIT | (1 << 2) | LDE_ROW_HALF;
-    return logN >= 12 ? (LDE_SPLIT | (2 << 2) | LDE_ROW_HALF) : (LDE_WIDE | (2 << 2));
+    // form with 2^13-element row tiles (inverse rows only at 2^12 rows; column
+    // tiles 2^12, 2^13 from 2^18 rows), the single fused kernel below 2^12 rows
+    if (logN >= 18) return LDE_SPLIT | (1 << 2) | LDE_ROW_HALF;
+    if (logN >= 13) return LDE_SPLIT | (2 << 2) | LDE_ROW_HALF;
+    if (logN == 12) return LDE_SPLIT | (1 << 2) | LDE_INV_HALF;
+    return LDE_WIDE | (2 << 2);
 }
 
 bool lde_variant_valid(int v) {
