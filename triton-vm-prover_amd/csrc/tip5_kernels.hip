@@ -65,26 +65,15 @@ __device__ __forceinline__ void hash_rows_body(const HashArgs& A, u64 blk, const
     store_digest(A.dig + row * tip5::DIGEST, s);
 }
 
-// At least 4 waves per SIMD (<= 128 VGPRs): the few registers this costs spill
-// outside the permutation loop, and the extra wave hides more latency.
-// Waves per SIMD the row-hash kernels are compiled for (VGPR budget 512 / occupancy).
-#ifndef TVH_HASH_OCC
-#define TVH_HASH_OCC 4
-#endif
-#ifdef TVH_HASH_WPE
-#define TVH_HASH_ATTR __attribute__((amdgpu_waves_per_eu(TVH_HASH_WPE, TVH_HASH_WPE)))
-#else
-#define TVH_HASH_ATTR
-#endif
-
-__global__ void TVH_HASH_ATTR __launch_bounds__(256, TVH_HASH_OCC) k_hash_rows(HashArgs A) {
+// At least 4 waves per SIMD (<= 128 VGPRs; 4 to 6 waves time the same).
+__global__ void __launch_bounds__(256, 4) k_hash_rows(HashArgs A) {
     __shared__ uint8_t lut[256];
     tip5::load_lut(lut);
     __syncthreads();
     hash_rows_body(A, blockIdx.x, lut);
 }
 
-__global__ void __launch_bounds__(256, TVH_HASH_OCC) k_hash_rows_multi(const HashArgs* __restrict__ args,
+__global__ void __launch_bounds__(256, 4) k_hash_rows_multi(const HashArgs* __restrict__ args,
                                                          const u32* __restrict__ start, u32 nseg) {
     __shared__ uint8_t lut[256];
     tip5::load_lut(lut);
