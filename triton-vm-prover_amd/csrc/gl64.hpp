@@ -25,15 +25,40 @@ constexpr u64 EPS = 0xFFFFFFFFull;  // 2^64 mod p
 
 __host__ __device__ __forceinline__ u64 mk64(u32 lo, u32 hi) { return ((u64)hi << 32) | lo; }
 
+// Starts a carry chain for the device compiler: the value is pinned in a VGPR
+// as it stands, so the chain that produced it cannot be merged into the next.
+// (The gfx950 compiler of ROCm 7.2 folds a final subtract-with-borrow of 0 into
+// a following add-with-carry of 0, as in canon(montyred(..)) or canon(sub(..)),
+// and then selects on a wrong carry; the device check
+// tests/native/gl64_device_check.hip shows it.)  canon() pins its input.  No
+// instruction is emitted; a no-op on the host.
+__host__ __device__ __forceinline__ void opaque(u32& x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("" : "+v"(x));
+#else
+    (void)x;
+#endif
+}
+
 // Explicit 32-bit carry chains (v_add_co/v_addc, v_sub_co/v_subb on gfx950):
 // full-rate VALU ops, no 64-bit compares.  The same code serves the host.
 
-// any u64 -> canonical
-__host__ __device__ __forceinline__ u64 canon(u64 x) {
+// any u64 -> canonical, for an input whose producer is known to compile
+// correctly in front of this chain (a 64-bit fold, a load)
+__host__ __device__ __forceinline__ u64 canon_raw(u64 x) {
     u32 k, c;
     u32 t0 = __builtin_addc((u32)x, 0xFFFFFFFFu, 0u, &k);  // x + EPS carries out iff x >= p
     u32 t1 = __builtin_addc((u32)(x >> 32), 0u, k, &c);
     return c ? mk64(t0, t1) : x;
+}
+
+// any u64 -> canonical.  The input is pinned (opaque) so that the carry chain
+// that produced it is never merged into this one.
+__host__ __device__ __forceinline__ u64 canon(u64 x) {
+    u32 x0 = (u32)x, x1 = (u32)(x >> 32);
+    opaque(x0);
+    opaque(x1);
+    return canon_raw(mk64(x0, x1));
 }
 
 // a, b canonical -> canonical
@@ -207,7 +232,7 @@ __host__ __device__ __forceinline__ u64 mul_lazy(u64 a, u64 b) {
 }
 
 // a, b any u64 -> canonical
-__host__ __device__ __forceinline__ u64 mul(u64 a, u64 b) { return canon(mul_lazy(a, b)); }
+__host__ __device__ __forceinline__ u64 mul(u64 a, u64 b) { return canon_raw(mul_lazy(a, b)); }
 
 // 2^e mod p for 0 <= e < 192 (compile-time usable)
 __host__ __device__ constexpr u64 pow2_mod_p(int e) {
@@ -258,18 +283,37 @@ __host__ __device__ __forceinline__ u64 mul_pow2(u64 x) {
         w[q + 1] = (x1 << r) | (x0 >> (32 - r));
         w[q + 2] = x1 >> (32 - r);
     }
-    u64 lo = mk64(w[0], w[1]);
-    u64 hi = mk64(w[3], w[4]);
-    // t = lo + w2 * (2^32 - 1): < 2^64 + 2^64, one carry folded back as 2^64 = EPS
-    u64 m = (u64)w[2] * EPS;
-    u64 t = lo + m;
-    if (t < m) t += EPS;  // cannot carry again: t < 2^64 - 2^33 here
-    if (q > 0) {          // hi = 0 when E < 32
-        u64 u = t - hi;
-        if (t < hi) u -= EPS;  // borrow: add p (u >= 2^63 > EPS before the fix)
-        t = u;
+    // In 32-bit words (no 64-bit multiply or compare):
+    // m = w2 * (2^32 - 1) = (w2 - [w2 != 0]) : (-w2)
+    u32 b0, b1, c0, c1, c2, c3;
+    u32 m0 = __builtin_subc(0u, w[2], 0u, &b0);
+    u32 m1 = __builtin_subc(w[2], 0u, b0, &b1);
+    // t = lo + m < 2^65 - 2^33; a carry (2^64) folds back as EPS and cannot
+    // carry again (t mod 2^64 <= 2^64 - 2^33 then)
+    u32 t0 = __builtin_addc(w[0], m0, 0u, &c0);
+    u32 t1 = __builtin_addc(w[1], m1, c0, &c1);
+    t0 = __builtin_addc(t0, c1 ? 0xFFFFFFFFu : 0u, 0u, &c2);
+    t1 = __builtin_addc(t1, 0u, c2, &c3);
+    // v = t - hi, hi = (w4:w3) = x * 2^E >> 96 < 2^63 (0 when E < 32), so
+    // -2^63 < v < 2^64; the canonical result is one of u = v mod 2^64, u + EPS
+    // (v >= p: minus p) and u - EPS (v < 0: plus p, which lands in (0, p)),
+    // selected at the end (no borrow fix feeding a second carry chain: the
+    // gfx950 compiler merges such chains incorrectly, see opaque())
+    u32 d1 = 0, u0 = t0, u1 = t1;
+    if (q > 0) {
+        u32 d0;
+        u0 = __builtin_subc(t0, w[3], 0u, &d0);
+        u1 = __builtin_subc(t1, w[4], d0, &d1);
     }
-    return canon(t);
+    u32 k0, k1, g0, g1;
+    u32 a0 = __builtin_addc(u0, 0xFFFFFFFFu, 0u, &k0);  // u + EPS carries iff u >= p
+    u32 a1 = __builtin_addc(u1, 0u, k0, &k1);
+    u32 s0 = __builtin_subc(u0, 0xFFFFFFFFu, 0u, &g0);  // u - EPS (mod 2^64)
+    u32 s1 = __builtin_subc(u1, 0u, g0, &g1);
+    (void)b1;
+    (void)c3;
+    (void)g1;
+    return d1 ? mk64(s0, s1) : (k1 ? mk64(a0, a1) : mk64(u0, u1));
 }
 
 // x canonical, k runtime in [0, 192) -> canonical x * 2^k (host and tables)

@@ -348,6 +348,12 @@ __device__ __forceinline__ u32 find_segment(const u32* __restrict__ start, u32 n
 }
 
 // Transforms s = (c, n2): element i at src[c*sstride + (i << b) + n2].
+// Addressing in the column and row functors: every access is a block-uniform
+// base (the first transform of the block) plus a 32-bit element offset, so the
+// loads and stores take a scalar base and a vector offset and no 64-bit
+// address arithmetic runs per element.  Offsets stay below 2^32 elements: a
+// block's transforms span at most its tile's worth of columns, and the
+// in-column offset is below M <= 2^27.
 struct Step1Load {
     const u64* src;
     u64 sstride, s0, ntr;
@@ -356,9 +362,11 @@ struct Step1Load {
     __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
         u64 s = s0 + t;
         if (s < ntr) {
-            const u64* p = src + (s >> b) * sstride + (s & ((1ull << b) - 1));
+            const u64 mask = (1ull << b) - 1;
+            const u64* base = src + (s0 >> b) * sstride + (s0 & mask);
+            const u32 o = (u32)(((s >> b) - (s0 >> b)) * sstride + (s & mask) - (s0 & mask));
 #pragma unroll
-            for (int r = 0; r < R; r++) v[r] = p[(u64)(j + r * per) << b];
+            for (int r = 0; r < R; r++) v[r] = base[o + ((u32)(j + r * per) << b)];
         } else {
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = 0;
@@ -376,8 +384,10 @@ struct Step1Store {
     __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
         u64 s = s0 + t;
         if (s >= ntr) return;
-        u64 n2 = s & ((1ull << b) - 1);
-        u64* p = dst + (s >> b) * dstride + n2;
+        const u64 mask = (1ull << b) - 1;
+        u64 n2 = s & mask;
+        u64* p0 = dst + (s0 >> b) * dstride + (s0 & mask);
+        const u32 o = (u32)(((s >> b) - (s0 >> b)) * dstride + n2 - (s0 & mask));
         int k = j & (Ns - 1), base = (j - k) * R + k;
         // table entries first (index 0 holds 1, so n2 = 0 or k1 = 0 needs no
         // branch), then the products: the loads are in flight together
@@ -392,7 +402,7 @@ struct Step1Store {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             u32 k1 = (u32)(base + r * Ns);
-            p[((u64)E * k1) << b] = gl::mont_mul(v[r], gl::mont_mul(tl[r], th[r]));
+            p0[o + ((E * k1) << b)] = gl::mont_mul(v[r], gl::mont_mul(tl[r], th[r]));
         }
     }
 };
@@ -575,9 +585,11 @@ struct RowLoad {
     __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
         u64 s = s0 + t;
         if (s < nrows) {
-            const u64* p = src + (s >> a) * scs + (s & ((1ull << a) - 1)) * srs;
+            const u64 mask = (1ull << a) - 1;
+            const u64* p = src + (s0 >> a) * scs + (s0 & mask) * srs;
+            const u32 o = (u32)(((s >> a) - (s0 >> a)) * scs + ((s & mask) - (s0 & mask)) * srs);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[r] = p[j + r * per];
+            for (int r = 0; r < R; r++) v[r] = p[o + (u32)(j + r * per)];
         } else {
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = 0;
@@ -593,10 +605,12 @@ struct RowStore {
     __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
         u64 s = s0 + t;
         if (s >= nrows) return;
-        u64* p = out + (s >> a) * M + (((u64)E * (s & ((1ull << a) - 1))) << b);
+        const u64 mask = (1ull << a) - 1;
+        u64* p = out + (s0 >> a) * M + (((u64)E * (s0 & mask)) << b);
+        const u32 o = (u32)(((s >> a) - (s0 >> a)) * M + (((u64)E * ((s & mask) - (s0 & mask))) << b));
         int k = j & (Ns - 1), base = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; r++) p[base + r * Ns] = v[r];
+        for (int r = 0; r < R; r++) p[o + (u32)(base + r * Ns)] = v[r];
     }
 };
 
@@ -646,16 +660,18 @@ struct ExpandLoad {
             for (int r = 0; r < R; r++) v[r] = 0;
             return;
         }
-        u64 c = s >> a, k1 = s & ((1ull << a) - 1);
-        const u64* p = A.src + c * A.src_col_stride + k1 * A.src_row_stride;
-        const u64* pb = A.pb + ((u64)sl << b);
+        const u64 mask = (1ull << a) - 1;
+        u64 c = s >> a, k1 = s & mask;
+        const u64* p = A.src + (s0 >> a) * A.src_col_stride + (s0 & mask) * A.src_row_stride;
+        const u32 o = (u32)((c - (s0 >> a)) * A.src_col_stride + (k1 - (s0 & mask)) * A.src_row_stride);
+        const u32 ob = sl << b;
         // every load first, with no branch in between, so that they are in
         // flight together
         u64 f[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            v[r] = p[j + r * per];
-            f[r] = pb[j + r * per];
+            v[r] = p[o + (u32)(j + r * per)];
+            f[r] = A.pb[ob + (u32)(j + r * per)];
         }
         // randomizer terms: element k2 of row k1 is trace row k1 + k2 * N1,
         // which carries one when below rand_len (possible only for k1 < rand_len)
@@ -683,24 +699,27 @@ struct ExpandStore {
         u32 sl = (u32)T & (E - 1);
         u64 s = s0 + ((u32)T >> A.logE);
         if (s >= nrows) return;
-        u64 c = s >> a, k1 = s & ((1ull << a) - 1);
-        u64* dst = A.out + c * A.M + (((u64)E * k1) << b) + sl;
+        u64 k1 = s & ((1ull << a) - 1);
+        // row s starts at out + s*E*N2 (c*M + k1*E*N2 with M = N1*E*N2)
+        u64* dst = A.out + ((s0 * E) << b);
+        const u32 o = (u32)(((s - s0) * E) << b) + sl;
+        // (twiddle index 0 holds 1: no branch for k1 = 0 or u1 = 0)
         u64 f = A.pa[((u64)sl << a) + k1];
         int k = j & (Ns - 1), base = (j - k) * R + k;
         if (R * Ns == (1 << b)) {  // base = j: u1 = j + Ns*r
-            if (k1 && j) f = gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * j), A.logN));
+            f = gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * j), A.logN));
             const u64 step = twiddle(rt, 0, (u32)(k1 * Ns), A.logN);
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                dst[(u64)E * (j + r * Ns)] = gl::mont_mul(v[r], f);
+                dst[o + E * (u32)(j + r * Ns)] = gl::mont_mul(v[r], f);
                 if (r + 1 < R) f = gl::mont_mul(f, step);
             }
         } else {
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 u32 u1 = (u32)(base + r * Ns);
-                u64 g = (u1 && k1) ? gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * u1), A.logN)) : f;
-                dst[(u64)E * u1] = gl::mont_mul(v[r], g);
+                u64 g = gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * u1), A.logN));
+                dst[o + E * u1] = gl::mont_mul(v[r], g);
             }
         }
     }
@@ -753,13 +772,21 @@ struct Step3IO {
     u64* out;
     u64 M, s0, ntr;
     u32 logQ;
+    __device__ __forceinline__ u64* base() const {
+        return out + (s0 >> logQ) * M + (s0 & ((1ull << logQ) - 1));
+    }
+    __device__ __forceinline__ u32 off(u64 s) const {
+        const u64 mask = (1ull << logQ) - 1;
+        return (u32)(((s >> logQ) - (s0 >> logQ)) * M + (s & mask) - (s0 & mask));
+    }
     template <int R>
     __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
         u64 s = s0 + t;
         if (s < ntr) {
-            const u64* p = out + (s >> logQ) * M + (s & ((1ull << logQ) - 1));
+            const u64* p = base();
+            const u32 o = off(s);
 #pragma unroll
-            for (int r = 0; r < R; r++) v[r] = p[(u64)(j + r * per) << logQ];
+            for (int r = 0; r < R; r++) v[r] = p[o + ((u32)(j + r * per) << logQ)];
         } else {
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = 0;
@@ -769,10 +796,11 @@ struct Step3IO {
     __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
         u64 s = s0 + t;
         if (s >= ntr) return;
-        u64* p = out + (s >> logQ) * M + (s & ((1ull << logQ) - 1));
+        u64* p = base();
+        const u32 o = off(s);
         int k = j & (Ns - 1), base = (j - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; r++) p[(u64)(base + r * Ns) << logQ] = v[r];
+        for (int r = 0; r < R; r++) p[o + ((u32)(base + r * Ns) << logQ)] = v[r];
     }
 };
 
