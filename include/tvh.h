@@ -139,11 +139,14 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t t
  * gives the two row kernels 2^13-element tiles (512 threads, two blocks per CU)
  * instead of 2^14, with the row length halved where E * row length would not
  * fit; TVH_LDE_INV_HALF (with TVH_LDE_SPLIT only, not with TVH_LDE_ROW_HALF)
- * gives only the inverse row kernel the 2^13 tile.  The variant is chosen per
+ * gives only the inverse row kernel the 2^13 tile; TVH_LDE_ROW_QUARTER (with
+ * TVH_LDE_SPLIT only, alone) gives both row kernels 2^12-element tiles (256
+ * threads, four blocks per CU).  The variant is chosen per
  * (log2 trace_len, log2 expansion); TVH_LDE_AUTO restores the default
- * (TVH_LDE_SPLIT | TVH_LDE_ROW_HALF with tile 2 from 2^13 rows on, tile 1 from
- * 2^18; TVH_LDE_SPLIT | TVH_LDE_INV_HALF with tile 1 at 2^12; TVH_LDE_WIDE with
- * tile 2 below).  Batch plans use each table's form with column tile 2 and
+ * (TVH_LDE_SPLIT with column tile 2 and TVH_LDE_ROW_QUARTER for 2^13..2^16
+ * rows, TVH_LDE_ROW_HALF for 2^17..2^18, column tile 1 and TVH_LDE_ROW_HALF
+ * from 2^19; TVH_LDE_SPLIT | TVH_LDE_INV_HALF with tile 1 at 2^12;
+ * TVH_LDE_WIDE with tile 2 below).  Batch plans use each table's form with column tile 2 and
  * 2^13-element row tiles.
  * No reference counterpart (the reference has one fixed kernel schedule per
  * size, src/gpu/kernels/ntt_kernel.cu). */
@@ -152,7 +155,8 @@ enum {
     TVH_LDE_WIDE = 1,
     TVH_LDE_SPLIT = 2,
     TVH_LDE_ROW_HALF = 16,
-    TVH_LDE_INV_HALF = 32
+    TVH_LDE_INV_HALF = 32,
+    TVH_LDE_ROW_QUARTER = 64
 };
 int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant);
 int tvh_lde_get_variant(uint32_t log_n, uint32_t log_e, int* variant);
@@ -161,7 +165,8 @@ int tvh_lde_variant_count(void);
 /* Times every variant on scratch buffers of this shape (num_cols capped so
  * the codewords stay under 4 GiB), keeps the fastest for the shape and returns
  * its code in *chosen; ms (may be NULL) receives the average milliseconds per
- * LDE of the variants 1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42 in that order.  Synchronous; allocates
+ * LDE of the variants 1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42, 66, 70, 74 in
+ * that order.  Synchronous; allocates
  * and frees its own buffers; not to be called concurrently with other work. */
 int tvh_lde_autotune(size_t num_cols, size_t trace_len, size_t target_len, int* chosen,
                      float* ms);

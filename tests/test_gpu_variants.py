@@ -1,6 +1,7 @@
 """Every LDE kernel variant (row-step form TVH_LDE_WIDE / TVH_LDE_SPLIT times the
 column-step tile 2^14 / 2^13 / 2^12, and the split form with 2^13-element tiles
-for both row kernels or the inverse rows only) against the oracle on every shape class (rows
+for both row kernels or the inverse rows only, or 2^12-element row tiles) against
+the oracle on every shape class (rows
 shorter than, equal to and split from 4096; expansions 1..16; randomizers; trace
 offsets), in single-table calls and inside batch plans, and the autotuner's contract."""
 import numpy as np
@@ -10,9 +11,10 @@ from test_gpu_parity import LDE_CASES
 
 pytestmark = pytest.mark.gpu
 
-WIDE, SPLIT, HALF, IHALF = 1, 2, 16, 32
-# form | tile code << 2 | half row tiles (both row kernels, or the inverse rows only)
-VARIANTS = (1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42)
+WIDE, SPLIT, HALF, IHALF, QUARTER = 1, 2, 16, 32, 64
+# form | tile code << 2 | smaller row tiles (2^13 for both row kernels or the inverse
+# rows only, 2^12 for both)
+VARIANTS = (1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42, 66, 70, 74)
 EXTRA = [(20, 4, 0, 1), (18, 2, 5, 3), (17, 16, 0, 1), (19, 1, 0, 1)]
 
 
@@ -76,7 +78,8 @@ def test_batch_plan_with_both_forms(T, oracle, variant):
     """A batch whose tables use different forms equals per-table commits."""
     shapes = [(10, 5, 4, WIDE), (12, 3, 4, SPLIT), (13, 7, 2, SPLIT | 4), (8, 11, 8, WIDE | 8),
               (14, 2, 4, SPLIT | 8), (11, 4, 4, SPLIT), (15, 3, 2, WIDE | 4),
-              (16, 2, 4, SPLIT | HALF), (9, 3, 16, SPLIT | 4 | HALF), (17, 2, 4, SPLIT | IHALF)]
+              (16, 2, 4, SPLIT | HALF), (9, 3, 16, SPLIT | 4 | HALF), (17, 2, 4, SPLIT | IHALF),
+              (18, 2, 4, SPLIT | QUARTER), (6, 5, 16, SPLIT | 8 | QUARTER)]
     tabs = []
     for i, (lg, w, E, form) in enumerate(shapes):
         variant(lg, E, form)
@@ -97,7 +100,7 @@ def test_batch_plan_with_both_forms(T, oracle, variant):
 def test_variant_codes(T, tvhmod):
     """The library accepts exactly the listed codes (and 0, the default)."""
     assert T.lib.lib.tvh_lde_variant_count() == len(VARIANTS)
-    for v in range(64):
+    for v in range(160):
         ok = v == 0 or v in VARIANTS
         if ok:
             T.lib.lde_set_variant(13, 2, v)

@@ -203,8 +203,8 @@ int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant) {
     return set_lde_variant(log_n, log_e, variant);
 }
 
-// form | column tile << 2 | LDE_ROW_HALF (16) or LDE_INV_HALF (32)
-static const int kLdeVariants[] = {1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42};
+// form | column tile << 2 | LDE_ROW_HALF (16), LDE_INV_HALF (32) or LDE_ROW_QUARTER (64)
+static const int kLdeVariants[] = {1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42, 66, 70, 74};
 static constexpr int kNumLdeVariants = sizeof(kLdeVariants) / sizeof(kLdeVariants[0]);
 
 int tvh_lde_variant_count(void) { return kNumLdeVariants; }

@@ -883,19 +883,22 @@ static std::atomic<int> g_variant[kMaxLogN + 1][5];
 int lde_variant(u32 logN, u32 logE) {
     int v = (logN <= kMaxLogN && logE <= 4) ? g_variant[logN][logE].load() : 0;
     if (v) return v;
-    // default (measured on MI355X at widths 156..379, 2^12..2^20 rows): the split
-    // form with 2^13-element row tiles (inverse rows only at 2^12 rows; column
-    // tiles 2^12, 2^13 from 2^18 rows), the single fused kernel below 2^12 rows
-    if (logN >= 18) return LDE_SPLIT | (1 << 2) | LDE_ROW_HALF;
-    if (logN >= 13) return LDE_SPLIT | (2 << 2) | LDE_ROW_HALF;
+    // default (measured on MI355X at widths 156..379, 2^12..2^21 rows): the split
+    // form with 2^12-element row tiles for 2^13..2^16 rows, 2^13-element row
+    // tiles above (column tiles 2^12, 2^13 from 2^19 rows), 2^13 for the inverse
+    // rows only at 2^12 rows, the single fused kernel below
+    if (logN >= 19) return LDE_SPLIT | (1 << 2) | LDE_ROW_HALF;
+    if (logN >= 17) return LDE_SPLIT | (2 << 2) | LDE_ROW_HALF;
+    if (logN >= 13) return LDE_SPLIT | (2 << 2) | LDE_ROW_QUARTER;
     if (logN == 12) return LDE_SPLIT | (1 << 2) | LDE_INV_HALF;
     return LDE_WIDE | (2 << 2);
 }
 
 bool lde_variant_valid(int v) {
-    int form = v & 3, tile = (v >> 2) & 3, half = (v >> 4) & 1, ihalf = v >> 5;
-    return v == LDE_AUTO || ((form == LDE_WIDE || form == LDE_SPLIT) && tile <= 2 &&
-                             half + ihalf <= (form == LDE_SPLIT));
+    int form = v & 3, tile = (v >> 2) & 3, half = (v >> 4) & 1, ihalf = (v >> 5) & 1,
+        quarter = (v >> 6) & 1;
+    return v == LDE_AUTO || (v < 128 && (form == LDE_WIDE || form == LDE_SPLIT) && tile <= 2 &&
+                             half + ihalf + quarter <= (form == LDE_SPLIT));
 }
 
 int set_lde_variant(u32 logN, u32 logE, int v) {
@@ -999,8 +1002,13 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
     // multi-table launches run the column kernels with the 2^12 tile and the
     // row kernels with the full tile
     P->col_lt = batch ? kBatchColLt : LOG_TILE - ((P->k2_variant >> 2) & 3);
-    P->row_lt = batch ? kBatchRowLt : (P->k2_variant & LDE_ROW_HALF) ? LOG_TILE - 1 : LOG_TILE;
-    P->inv_lt = (P->row_lt < LOG_TILE || (P->k2_variant & LDE_INV_HALF)) ? LOG_TILE - 1 : LOG_TILE;
+    P->row_lt = batch                                  ? kBatchRowLt
+                : (P->k2_variant & LDE_ROW_QUARTER) ? LOG_TILE - 2
+                : (P->k2_variant & LDE_ROW_HALF)    ? LOG_TILE - 1
+                                                     : LOG_TILE;
+    P->inv_lt = P->row_lt < LOG_TILE                 ? P->row_lt
+                : (P->k2_variant & LDE_INV_HALF)    ? LOG_TILE - 1
+                                                     : LOG_TILE;
     if ((P->k2_variant & 3) != LDE_SPLIT) P->row_lt = LOG_TILE;  // the wide kernel's tile
     P->s1_blocks = P->s2_blocks = P->s3_blocks = 0;
     if (cols == 0) return TVH_OK;
@@ -1116,15 +1124,21 @@ int launch_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, 
         if (P.inv_lt == LOG_TILE)
             hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE>, dim3(P.inv_blocks), itpb, P.inv_lds, st,
                                P.s2inv, ds->roots);
-        else
+        else if (P.inv_lt == LOG_TILE - 1)
             hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE - 1>, dim3(P.inv_blocks), itpb, P.inv_lds, st,
+                               P.s2inv, ds->roots);
+        else
+            hipLaunchKernelGGL(k_lde_rowinv<LOG_TILE - 2>, dim3(P.inv_blocks), itpb, P.inv_lds, st,
                                P.s2inv, ds->roots);
         if ((rc = check_launch())) return rc;
         if (P.row_lt == LOG_TILE)
             hipLaunchKernelGGL(k_lde_expand<LOG_TILE>, dim3(P.s2_blocks), rtpb, P.s2_lds, st, P.s2,
                                ds->roots);
-        else
+        else if (P.row_lt == LOG_TILE - 1)
             hipLaunchKernelGGL(k_lde_expand<LOG_TILE - 1>, dim3(P.s2_blocks), rtpb, P.s2_lds, st,
+                               P.s2, ds->roots);
+        else
+            hipLaunchKernelGGL(k_lde_expand<LOG_TILE - 2>, dim3(P.s2_blocks), rtpb, P.s2_lds, st,
                                P.s2, ds->roots);
     } else
         hipLaunchKernelGGL(k_lde_step2, dim3(P.s2_blocks), dim3(TPB), P.s2_lds, st, P.s2,
@@ -1244,7 +1258,8 @@ int ntt_set_attributes() {
                          (const void*)k_lde_step2_multi,
                          (const void*)k_lde_step3_multi<kBatchColLt>, (const void*)k_lde_rowinv<LOG_TILE>,
                          (const void*)k_lde_expand<LOG_TILE>, (const void*)k_lde_rowinv<LOG_TILE - 1>,
-                         (const void*)k_lde_expand<LOG_TILE - 1>, (const void*)k_lde_rowinv_multi<kBatchRowLt>,
+                         (const void*)k_lde_expand<LOG_TILE - 1>, (const void*)k_lde_rowinv<LOG_TILE - 2>,
+                         (const void*)k_lde_expand<LOG_TILE - 2>, (const void*)k_lde_rowinv_multi<kBatchRowLt>,
                          (const void*)k_lde_expand_multi<kBatchRowLt>,
 };
     for (const void* f : fns)

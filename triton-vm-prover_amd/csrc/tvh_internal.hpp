@@ -118,8 +118,16 @@ struct LdeArgs {
 // (LDE_SPLIT only) halves the row kernels' tile to 2^13 elements (512 threads,
 // two blocks per CU), which also halves the row length N2 when E * N2 would
 // not fit (all cosets of a row stay in one block); bit 5 (LDE_SPLIT only, not
-// with bit 4) gives only the inverse row kernel the 2^13 tile.
-enum LdeVariant { LDE_AUTO = 0, LDE_WIDE = 1, LDE_SPLIT = 2, LDE_ROW_HALF = 16, LDE_INV_HALF = 32 };
+// with bit 4) gives only the inverse row kernel the 2^13 tile; bit 6 (LDE_SPLIT
+// only, alone) gives both row kernels 2^12-element tiles (256 threads).
+enum LdeVariant {
+    LDE_AUTO = 0,
+    LDE_WIDE = 1,
+    LDE_SPLIT = 2,
+    LDE_ROW_HALF = 16,
+    LDE_INV_HALF = 32,
+    LDE_ROW_QUARTER = 64
+};
 int lde_variant(u32 logN, u32 logE);            // the tuned choice (never LDE_AUTO)
 bool lde_variant_valid(int v);
 int set_lde_variant(u32 logN, u32 logE, int v);  // v = LDE_AUTO restores the default

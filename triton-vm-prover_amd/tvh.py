@@ -73,13 +73,15 @@ JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 LDE_AUTO, LDE_WIDE, LDE_SPLIT = 0, 1, 2
 LDE_ROW_HALF = 16
 LDE_INV_HALF = 32
-# form | column tile code << 2 | LDE_ROW_HALF or LDE_INV_HALF (split form only)
-LDE_VARIANTS = (1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42)
+LDE_ROW_QUARTER = 64
+# form | column tile code << 2 | LDE_ROW_HALF, LDE_INV_HALF or LDE_ROW_QUARTER (split form only)
+LDE_VARIANTS = (1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42, 66, 70, 74)
 
 
 def lde_variant_name(v):
     name = {LDE_WIDE: "wide", LDE_SPLIT: "split"}[v & 3] + f"/t{14 - ((v >> 2) & 3)}"
-    return name + ("/r13" if v & LDE_ROW_HALF else "") + ("/i13" if v & LDE_INV_HALF else "")
+    return (name + ("/r13" if v & LDE_ROW_HALF else "") + ("/i13" if v & LDE_INV_HALF else "")
+            + ("/r12" if v & LDE_ROW_QUARTER else ""))
 
 _SIGS = {
     "tvh_init": [_i32],
