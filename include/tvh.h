@@ -146,8 +146,8 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t t
  * (TVH_LDE_SPLIT with column tile 2 and TVH_LDE_ROW_QUARTER for 2^13..2^16
  * rows, TVH_LDE_ROW_HALF for 2^17..2^18, column tile 1 and TVH_LDE_ROW_HALF
  * from 2^19; TVH_LDE_SPLIT | TVH_LDE_INV_HALF with tile 1 at 2^12;
- * TVH_LDE_WIDE with tile 2 below).  Batch plans use each table's form with column tile 2 and
- * 2^13-element row tiles.
+ * TVH_LDE_WIDE with tile 2 below).  Batch plans use each table's form with
+ * column tile 1 and 2^13-element row tiles.
  * No reference counterpart (the reference has one fixed kernel schedule per
  * size, src/gpu/kernels/ntt_kernel.cu). */
 enum {

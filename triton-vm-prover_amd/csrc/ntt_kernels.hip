@@ -999,8 +999,8 @@ int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff,
                 u64 m, bool truncate, u64* out, LdePlan* P, bool batch) {
     P->k1 = P->k3 = P->split = false;
     P->k2_variant = lde_variant(log2u(n), log2u(m / (n ? n : 1)));
-    // multi-table launches run the column kernels with the 2^12 tile and the
-    // row kernels with the full tile
+    // multi-table launches run the column and row kernels with one tile each
+    // (kBatchColLt, kBatchRowLt) for every table of the launch
     P->col_lt = batch ? kBatchColLt : LOG_TILE - ((P->k2_variant >> 2) & 3);
     P->row_lt = batch                                  ? kBatchRowLt
                 : (P->k2_variant & LDE_ROW_QUARTER) ? LOG_TILE - 2

@@ -161,8 +161,8 @@ enum LdePhase {
     LDE_PHASE_STEP3 = 4,   // k_lde_step3_multi
     LDE_PHASES = 5
 };
-constexpr u32 kBatchColLt = 12;  // column-kernel tile of multi-table launches
-constexpr u32 kBatchRowLt = 13;  // row-kernel tile of multi-table launches (2^13: 7 % faster than 2^14)
+constexpr u32 kBatchColLt = 13;  // column-kernel tile of multi-table launches
+constexpr u32 kBatchRowLt = 13;  // row-kernel tile of multi-table launches (measured best over 2^12..2^14)
 int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg, u32 blocks,
                      size_t lds, hipStream_t st);
 int launch_fill_uniform(u64* d, u64 count, u64 seed, hipStream_t st);
