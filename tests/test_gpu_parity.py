@@ -278,6 +278,38 @@ def test_commit_full_size_sampled(T, oracle):
             assert oracle.verify_path(root, r, leaf, T.host(out).reshape(-1, 5))
 
 
+def test_aux_commit_full_size_sampled(T, oracle):
+    """Aux-table shape 2^20 x 88 XFE (264 limb columns, limb-major x*3 + limb) -> 2^22: the
+    three limb columns of the first and last XFE column against the oracle's XFE-rule LDE,
+    sampled row digests against hash_varlen of the 264 limbs, root consistency."""
+    n, xc, E = 1 << 20, 88, 4
+    w, m = 3 * xc, E * n
+    t = oracle.table_rowmajor_draw(2, n, w)
+    dt = T.dev(t)
+    cols = {c: t[c * n:(c + 1) * n].copy() for c in (0, 1, 2, w - 3, w - 2, w - 1)}
+    del t
+    cw = T.empty(m * w)
+    tree = T.empty(5 * (2 * m - 1))
+    T.lib.lde_xfe_limbs(dt, xc, n, None, 0, 1, 7, m, cw, 0)
+    T.lib.hash_bfield_rows(cw, m, w, tree)
+    T.lib.merkle_tree(tree, tree, m, 0)
+    T.sync()
+    del dt
+    for c, x in cols.items():
+        assert np.array_equal(cw.download(m, c * m), oracle.lde_limb_truncating(x, m)), c
+    rng = np.random.default_rng(4)
+    rows = [0, m - 1] + [int(r) for r in rng.integers(0, m, 24)]
+    for r in rows:
+        row = cw.gather_strided(r, m, w)
+        assert np.array_equal(tree.download(5, 5 * r), oracle.hash_varlen(row)), r
+    # the root is the level-order root of the leaves the kernel wrote
+    leaves = tree.download(5 * m, 0).reshape(m, 5)
+    root = tree.download(5, 5 * (2 * m - 2))
+    small = T.empty(5)
+    T.lib.merkle_root(T.dev(leaves.reshape(-1)), small, m)
+    assert np.array_equal(T.host(small), root)
+
+
 # ------------------------------------------------------------------ streaming (frugal) commit
 @pytest.mark.parametrize("w,chunk,k", [(1, 1, 0), (9, 4, 0), (10, 3, 0), (23, 7, 2), (37, 10, 0),
                                        (64, 32, 0), (379, 32, 5), (21, 100, 0)])
