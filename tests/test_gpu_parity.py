@@ -328,6 +328,27 @@ def test_commit_frugal_matches_plain(T, oracle, w, chunk, k):
     assert np.array_equal(T.host(tree2).reshape(-1, 5)[-1], root)
 
 
+def test_commit_frugal_full_size(T, oracle):
+    """Frugal-mode shape 2^21 x 379 -> 2^23 in 32-column chunks: the whole tree (every digest
+    and the root) equals the plain commit's, whose first codeword column is checked against the
+    oracle."""
+    n, w, E, chunk = 1 << 21, 379, 4, 32
+    m = E * n
+    dt = T.empty(n * w)
+    T.lib.fill_uniform(dt, n * w, 21)
+    cw = T.empty(m * w)
+    tree = T.empty(5 * (2 * m - 1))
+    T.lib.commit(dt, w, n, None, 0, 1, 7, m, cw, tree, 0)
+    T.sync()
+    assert np.array_equal(cw.download(m, 0), oracle.lde_column(dt.download(n, 0), m))
+    del cw
+    work = T.empty(T.lib.frugal_workspace_elems(w, n, m, chunk))
+    tree2 = T.empty(5 * (2 * m - 1))
+    T.lib.commit_frugal(dt, w, n, None, 0, 1, 7, m, chunk, work, tree2, 0)
+    T.sync()
+    assert np.array_equal(T.host(tree2), T.host(tree))
+
+
 def test_commit_frugal_heap_layout(T, oracle):
     n, w, m = 32, 19, 128
     t = oracle.table_rowmajor_draw(5, n, w)
