@@ -22,10 +22,18 @@
  *
  * `_dev` calls take device pointers and a hipStream_t passed as void*
  * (NULL = the default stream); they are asynchronous on that stream, never
- * synchronise it and allocate no device memory, except where stated
- * (tvh_ntt_*_batched_dev and tvh_merkle_root_dev use a per-device arena that
- * grows on first use of a larger size).  `_host` calls take host pointers and
- * are synchronous.
+ * synchronise it and allocate no device memory, except where stated:
+ *  - tvh_ntt_*_batched_dev, tvh_lde_xfe_batch_dev and tvh_merkle_root_dev use
+ *    a per-device scratch arena that grows (waiting for the device, then
+ *    reallocating) on first use of a larger size; calls that use it must be
+ *    ordered on one stream;
+ *  - every LDE (tvh_lde_*_dev, tvh_commit_dev, tvh_commit_frugal_dev) reads
+ *    per-device coset tables for its (trace_len, target_len, trace_offset,
+ *    target_offset); the first call with a new key builds them (one small
+ *    allocation and a wait on a private stream).  tvh_lde_prepare builds them
+ *    ahead of time, so that later calls allocate nothing and never wait.
+ * `_host` calls take host pointers, are synchronous and use device buffers of
+ * their own (calls from several threads at once share nothing on the device).
  */
 #ifndef TVH_H
 #define TVH_H
@@ -126,6 +134,17 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* d_limbs, size_t num_xfe_cols, size_t t
                           uint64_t target_offset, size_t target_len, uint64_t* d_out,
                           int use_xfe_randomizers, void* stream);
 
+/* Builds the per-device coset tables every LDE of this shape and these
+ * offsets reads (at most E * (N1 + N2) u64 each, N1 * N2 = trace_len), so
+ * that later LDE calls with the same key allocate nothing and never wait.
+ * Synchronous; idempotent.  At most 256 keys are kept (TVH_ERR_RANGE beyond;
+ * tvh_lde_release_constants frees them all).  No reference counterpart (the
+ * reference recomputes its coset powers per call, randomized_lde_kernel.cu:990-1050). */
+int tvh_lde_prepare(size_t trace_len, size_t target_len, uint64_t trace_offset,
+                    uint64_t target_offset);
+/* Waits for the device, then frees every cached coset table. */
+int tvh_lde_release_constants(void);
+
 /* ---- LDE kernel forms and autotuning -------------------------------------- */
 
 /* Kernel variants of the LDE, all with identical results.  A variant code is
@@ -186,6 +205,15 @@ int tvh_hash_xfield_rows_dev(const uint64_t* d_table, size_t num_rows, size_t nu
 /* In-place Tip5 permutation of num_states 16-element states.
  * Replaces Backend::tip5_permutation_batch (include/backend/backend.hpp:95-101). */
 int tvh_tip5_permutation_batch_dev(uint64_t* d_states, size_t num_states, void* stream);
+
+/* Rounds [first_round, first_round + num_rounds) of the permutation on
+ * num_states 16-element states, in place (first_round + num_rounds <= 5,
+ * TVH_ERR_RANGE otherwise).  d_trace (may be NULL) receives, per state, the
+ * input state and the state after each of those rounds:
+ * d_trace[(i * (num_rounds + 1) + j) * 16 + k].  Replaces Tip5::round and
+ * Tip5::trace (src/hash/tip5.cpp:98-125). */
+int tvh_tip5_rounds_dev(uint64_t* d_states, size_t num_states, uint32_t first_round,
+                        uint32_t num_rounds, uint64_t* d_trace, void* stream);
 
 /* out[i] = hash_pair(left[i], right[i]).  Replaces Backend::hash_pairs
  * (include/backend/backend.hpp:103-115) and hash_pairs_gpu (tip5_kernel.cu:210-248). */
@@ -406,6 +434,9 @@ int tvh_lde_xfe_batch_host(const uint64_t* xfe_trace, size_t num_cols, size_t tr
 
 /* Replace Backend::tip5_permutation_batch / hash_pairs on host buffers. */
 int tvh_tip5_permutation_batch_host(uint64_t* states, size_t num_states);
+/* tvh_tip5_rounds_dev on host states (trace may be NULL). */
+int tvh_tip5_rounds_host(uint64_t* states, size_t num_states, uint32_t first_round,
+                         uint32_t num_rounds, uint64_t* trace);
 int tvh_hash_pairs_host(const uint64_t* left, const uint64_t* right, uint64_t* out,
                         size_t count);
 

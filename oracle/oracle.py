@@ -52,7 +52,10 @@ class Oracle:
             "orc_lde_column": (i32, [_u64p, sz, u64, C.c_void_p, sz, u64, sz, _u64p]),
             "orc_lde_limb_truncating": (i32, [_u64p, sz, u64, C.c_void_p, sz, u64, sz, _u64p]),
             "orc_lde_table_colmajor": (i32, [_u64p, sz, sz, u64, C.c_void_p, sz, u64, sz, _u64p]),
+            "orc_lde_sample_colmajor": (i32, [_u64p, sz, sz, u64, C.c_void_p, sz, u64, sz,
+                                              _u64p, sz, i32, _u64p]),
             "orc_tip5_permutation": (None, [_u64p]),
+            "orc_tip5_round": (None, [_u64p, i32]),
             "orc_hash_10": (None, [_u64p, _u64p]),
             "orc_hash_varlen": (None, [_u64p, sz, _u64p]),
             "orc_hash_pair": (None, [_u64p, _u64p, _u64p]),
@@ -133,10 +136,28 @@ class Oracle:
             raise ValueError("bad lde arguments")
         return out
 
+    def lde_sample(self, trace_cm, n, cols, m, rows, trace_off=1, target_off=7, rand=None,
+                   truncate=False):
+        """LDE values of every column at the given rows: (cols, len(rows))."""
+        t = np.ascontiguousarray(trace_cm, dtype=np.uint64).reshape(-1)
+        rp, k, _keep = self._rand_ptr(None if rand is None else np.asarray(rand).reshape(cols, -1))
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        out = np.zeros(cols * r.size, np.uint64)
+        if self.lib.orc_lde_sample_colmajor(t, n, cols, trace_off, rp, k, target_off, m,
+                                            r if r.size else np.zeros(1, np.uint64), r.size,
+                                            int(truncate), out):
+            raise ValueError("bad lde arguments")
+        return out.reshape(cols, r.size)
+
     # Tip5 ---------------------------------------------------------------------
     def permutation(self, st):
         s = np.ascontiguousarray(st, dtype=np.uint64).copy()
         self.lib.orc_tip5_permutation(s)
+        return s
+
+    def tip5_round(self, st, r):
+        s = np.ascontiguousarray(st, dtype=np.uint64).copy()
+        self.lib.orc_tip5_round(s, r)
         return s
 
     def hash_10(self, x):
@@ -250,6 +271,7 @@ class Reference:
             "ref_lde_column": (i32, [_u64p, sz, u64, C.c_void_p, sz, u64, sz, _u64p]),
             "ref_lde_xfe_column": (i32, [_u64p, sz, u64, C.c_void_p, sz, i32, u64, sz, _u64p]),
             "ref_tip5_permutation": (None, [_u64p]),
+            "ref_tip5_trace": (None, [_u64p, _u64p]),
             "ref_hash_varlen": (None, [_u64p, sz, _u64p]),
             "ref_hash_pair": (None, [_u64p, _u64p, _u64p]),
             "ref_hash_rows_colmajor": (None, [_u64p, sz, sz, _u64p]),
@@ -294,6 +316,12 @@ class Reference:
                                          k, int(xfe_rand), target_off, m, out)
         assert rc == 0
         return out.reshape(m, 3)
+
+    def tip5_trace(self, st):
+        s = np.ascontiguousarray(st, dtype=np.uint64)
+        out = np.zeros(6 * 16, np.uint64)
+        self.lib.ref_tip5_trace(s, out)
+        return out.reshape(6, 16)
 
     def permutation(self, st):
         s = np.ascontiguousarray(st, dtype=np.uint64).copy()

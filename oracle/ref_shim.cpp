@@ -122,6 +122,15 @@ void ref_tip5_permutation(uint64_t* st) {
     for (size_t i = 0; i < 16; i++) st[i] = t.state[i].value();
 }
 
+// state before and after each round (Tip5::trace): out holds 6 x 16 u64
+void ref_tip5_trace(const uint64_t* st, uint64_t* out) {
+    Tip5 t;
+    for (size_t i = 0; i < 16; i++) t.state[i] = BFieldElement(st[i]);
+    auto tr = t.trace();
+    for (size_t j = 0; j < tr.size() && j < 6; j++)
+        for (size_t i = 0; i < 16; i++) out[16 * j + i] = tr[j][i].value();
+}
+
 void ref_hash_varlen(const uint64_t* in, size_t len, uint64_t* out) {
     from_digest(Tip5::hash_varlen(to_bfe(in, len)), out);
 }

@@ -210,6 +210,37 @@ int orc_lde_limb_truncating(const uint64_t* trace, size_t n, uint64_t toff,
     return 0;
 }
 
+/* LDE values at sampled rows only: out[c * nrows + q] = p_c(goff * w_m^rows[q]),
+ * with p_c the randomized interpolant of column c (the polynomial whose coset
+ * evaluation lde_randomized.cpp:71-154 computes, chunking included) or, with
+ * truncate, its first m coefficients (the XFE rule, :270-300).  Horner at each
+ * point; columns in parallel (OpenMP).  Returns 0, or -1 for bad lengths. */
+int orc_lde_sample_colmajor(const uint64_t* trace, size_t n, size_t cols, uint64_t toff,
+                            const uint64_t* rand, size_t k, uint64_t goff, size_t m,
+                            const uint64_t* rows, size_t nrows, int truncate, uint64_t* out) {
+    int lm = log2_exact(m);
+    if (log2_exact(n) < 0 || lm < 0) return -1;
+    if (!rand) k = 0;
+    uint64_t wm = orc_root_of_unity((uint32_t)lm);
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long c = 0; c < (long)cols; c++) {
+        size_t len;
+        uint64_t* p = randomized_interpolant(trace + (size_t)c * n, n, toff,
+                                             rand ? rand + (size_t)c * k : NULL, k, &len);
+        if (!p) { bad = 1; continue; }
+        size_t use = truncate && len > m ? m : len;
+        for (size_t q = 0; q < nrows; q++) {
+            uint64_t x = orc_mul(goff, orc_pow(wm, rows[q] % m));
+            uint64_t acc = 0;
+            for (size_t i = use; i-- > 0;) acc = orc_add(orc_mul(acc, x), p[i]);
+            out[(size_t)c * nrows + q] = acc;
+        }
+        free(p);
+    }
+    return bad ? -2 : 0;
+}
+
 int orc_lde_table_colmajor(const uint64_t* trace, size_t n, size_t cols, uint64_t toff,
                            const uint64_t* rand, size_t k, uint64_t goff, size_t m,
                            uint64_t* out) {
@@ -287,9 +318,9 @@ static uint64_t split_and_lookup(uint64_t v) {   /* tip5.cpp:155-179 */
     return orc_montyred(0, o) % ORC_P;
 }
 
-void orc_tip5_permutation(uint64_t st[16]) {      /* tip5.cpp:92-96,115-203 */
+void orc_tip5_round(uint64_t st[16], int r) {   /* tip5.cpp:115-125 */
     lut_init();
-    for (int r = 0; r < 5; r++) {
+    {
         for (int i = 0; i < 4; i++) st[i] = split_and_lookup(st[i]);   /* :121-126 */
         for (int i = 4; i < 16; i++) {                                  /* :128-132 */
             uint64_t sq = orc_mul(st[i], st[i]);
@@ -305,6 +336,10 @@ void orc_tip5_permutation(uint64_t st[16]) {      /* tip5.cpp:92-96,115-203 */
         }
         for (int i = 0; i < 16; i++) st[i] = orc_add(ns[i], RC[r * 16 + i]);  /* :198-203 */
     }
+}
+
+void orc_tip5_permutation(uint64_t st[16]) {      /* tip5.cpp:92-96 */
+    for (int r = 0; r < 5; r++) orc_tip5_round(st, r);
 }
 
 void orc_hash_10(const uint64_t in[10], uint64_t out[5]) {   /* tip5.cpp:224-244 */

@@ -63,6 +63,10 @@ int orc_lde_table_colmajor(const uint64_t* trace, size_t n, size_t cols,
                            uint64_t target_off, size_t m, uint64_t* out);
 
 /* ---- Tip5 (tip5.cpp) ---- */
+int orc_lde_sample_colmajor(const uint64_t* trace, size_t n, size_t cols, uint64_t toff,
+                            const uint64_t* rand, size_t k, uint64_t goff, size_t m,
+                            const uint64_t* rows, size_t nrows, int truncate, uint64_t* out);
+void orc_tip5_round(uint64_t st[16], int round_index);
 void orc_tip5_permutation(uint64_t st[16]);
 void orc_hash_10(const uint64_t in[10], uint64_t out[5]);
 void orc_hash_varlen(const uint64_t* in, size_t len, uint64_t out[5]);

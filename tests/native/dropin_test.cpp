@@ -1,16 +1,24 @@
-// Exercises the drop-in C++ classes (include/triton_vm_amd/*.hpp) on the GPU the
-// way the reference's own unit tests exercise its classes.  Prints one JSON
-// object per line; tests/test_gpu_dropin.py checks them against the golden
-// fixtures and the oracle.  Exit status is non-zero on an unexpected exception.
+// Exercises the drop-in C++ classes (the reference's include paths:
+// ntt/ntt.hpp, lde/*.hpp, hash/tip5.hpp, merkle/merkle_tree.hpp,
+// table/table_commitment.hpp, types/*.hpp) on the GPU the way the reference's
+// own unit tests exercise its classes.  Prints one JSON object per line;
+// tests/test_gpu_dropin.py checks them against the golden fixtures and the
+// oracle.  Exit status is non-zero on an unexpected exception.
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
-#include "triton_vm_amd/lde.hpp"
-#include "triton_vm_amd/merkle.hpp"
-#include "triton_vm_amd/ntt.hpp"
-#include "triton_vm_amd/tip5.hpp"
+#include "hash/tip5.hpp"
+#include "lde/lde_randomized.hpp"
+#include "lde/lde_table.hpp"
+#include "merkle/merkle_tree.hpp"
+#include "ntt/ntt.hpp"
+#include "table/table_commitment.hpp"
+#include "types/b_field_element.hpp"
+#include "types/digest.hpp"
+#include "types/x_field_element.hpp"
 
 using namespace triton_vm;
 
@@ -136,6 +144,125 @@ int main() {
                     MerkleTree::verify_authentication_structure(t.root(), 64, qi, ql, t.authentication_structure(qi)) ? "true" : "false");
         expect_throw<std::invalid_argument>("merkle_non_pow2_throws", [] { MerkleTree m(std::vector<Digest>(3)); });
         expect_throw<std::out_of_range>("merkle_leaf_range_throws", [&] { t.leaf(64); });
+
+        // Tip5 public tables, per-round states (Tip5::trace / Tip5::round)
+        {
+            std::vector<BFieldElement> lut, mds;
+            for (uint8_t v : Tip5::LOOKUP_TABLE) lut.push_back(BFieldElement(v));
+            for (int64_t v : Tip5::MDS_MATRIX_FIRST_COLUMN) mds.push_back(BFieldElement((uint64_t)v));
+            print_vec("tip5_lookup_table", lut);
+            print_vec("tip5_mds_first_column", mds);
+            print_vec("tip5_round_constants",
+                      std::vector<BFieldElement>(Tip5::ROUND_CONSTANTS.begin(), Tip5::ROUND_CONSTANTS.end()));
+            auto s0 = splitmix(77, 16);
+            Tip5 a;
+            for (size_t i = 0; i < 16; i++) a.state[i] = s0[i];
+            Tip5 b = a, c = a;
+            auto tr = a.trace();
+            std::vector<BFieldElement> flat;
+            for (const auto& st : tr) flat.insert(flat.end(), st.begin(), st.end());
+            print_vec("tip5_trace_77", flat);
+            bool rounds_match = true;
+            for (size_t r = 0; r < Tip5::NUM_ROUNDS; r++) {
+                b.round(r);
+                rounds_match = rounds_match && b.state == tr[r + 1];
+            }
+            c.permutation();
+            std::printf("{\"tip5_rounds_match_trace\": %s, \"tip5_trace_leaves_permuted\": %s}\n",
+                        rounds_match ? "true" : "false", (a.state == c.state && c.state == tr.back()) ? "true" : "false");
+            expect_throw<std::out_of_range>("tip5_round_range_throws", [&] { b.round(5); });
+        }
+
+        // Value types: field inverses, Montgomery import, hex round trip
+        {
+            auto v = splitmix(99, 30);
+            bool inv_ok = true;
+            std::vector<XFieldElement> xs;
+            for (size_t i = 0; i + 2 < v.size(); i += 3) {
+                XFieldElement x(v[i], v[i + 1], v[i + 2]);
+                xs.push_back(x);
+                inv_ok = inv_ok && (x * x.inverse()).is_one() && (x / x).is_one() &&
+                         x.pow(5) * x.pow(7) == x.pow(12);
+            }
+            auto xinv = XFieldElement::batch_inversion(xs);
+            for (size_t i = 0; i < xs.size(); i++) inv_ok = inv_ok && xinv[i] == xs[i].inverse();
+            auto binv = BFieldElement::batch_inversion(v);
+            for (size_t i = 0; i < v.size(); i++) inv_ok = inv_ok && (binv[i] * v[i]).is_one();
+            // Montgomery form of x is x * 2^64 mod p = x * (2^32 - 1)
+            bool raw_ok = true;
+            for (const auto& x : v)
+                raw_ok = raw_ok && BFieldElement::from_raw_u64((x * BFieldElement(BFieldElement::MONTGOMERY_R)).value()) == x;
+            Digest dg(v[0], v[1], v[2], v[3], v[4]);
+            std::printf("{\"field_inverses_ok\": %s, \"from_raw_u64_ok\": %s, \"hex_roundtrip\": %s, \"xfe_string\": \"%s\"}\n",
+                        inv_ok ? "true" : "false", raw_ok ? "true" : "false",
+                        Digest::from_hex(dg.to_hex()) == dg ? "true" : "false",
+                        XFieldElement(BFieldElement(1), BFieldElement(2), BFieldElement(3)).to_string().c_str());
+            expect_throw<std::domain_error>("xfe_zero_inverse_throws", [] { XFieldElement::zero().inverse(); });
+        }
+
+        // NTT::interpolate from 8 host threads at once (each call uses its own
+        // device buffers): every result must equal its single-threaded value
+        {
+            const int T = 8;
+            std::vector<std::vector<BFieldElement>> cols(T), got(T);
+            for (int t = 0; t < T; t++) cols[t] = splitmix(500 + t, size_t(1) << (8 + t % 5));
+            std::vector<std::thread> th;
+            bool ok = true;
+            std::vector<int> same(T, 1);
+            for (int t = 0; t < T; t++)
+                th.emplace_back([&, t] {
+                    got[t] = NTT::interpolate(cols[t]);
+                    for (int rep = 0; rep < 6; rep++)
+                        if (NTT::interpolate(cols[t]) != got[t]) same[t] = 0;
+                });
+            for (auto& x : th) x.join();
+            for (int t = 0; t < T; t++) ok = ok && same[t];
+            std::printf("{\"threaded_interpolate_stable\": %s}\n", ok ? "true" : "false");
+            for (int t = 0; t < T; t++) {
+                std::string key = "threaded_interpolate_" + std::to_string(t);
+                print_vec(key.c_str(), got[t]);
+            }
+        }
+
+        // TableCommitment over an extended 2^10 x 379 table, and from digests
+        {
+            size_t n = 1024, w = 379;
+            auto flat = splitmix(1, n * w);
+            std::vector<std::vector<BFieldElement>> table(n, std::vector<BFieldElement>(w));
+            for (size_t r = 0; r < n; r++)
+                for (size_t c = 0; c < w; c++) table[r][c] = flat[r * w + c];
+            auto ext = LDETable::extend_table_with_randomizers(
+                table, ArithmeticDomain::of_length(n),
+                ArithmeticDomain::of_length(4 * n).with_offset(BFieldElement(7)),
+                std::vector<std::vector<BFieldElement>>(w));
+            TableCommitment tc1 = TableCommitment::commit_bfield_rows(ext);
+            print_digests("table_commitment_1024_379", {tc1.root()});
+            std::vector<Digest> rows(ext.size());
+            for (size_t r = 0; r < ext.size(); r++) rows[r] = hash_bfield_row(ext[r]);
+            TableCommitment tc2 = TableCommitment::from_digests(rows);
+            auto p = tc2.authentication_path(1234);
+            std::printf("{\"table_commitment_from_digests_matches\": %s, \"table_commitment_rows\": %zu, "
+                        "\"table_commitment_verify\": %s, \"table_commitment_tree_root\": %s}\n",
+                        tc2.root() == tc1.root() ? "true" : "false", tc1.num_rows(),
+                        TableCommitment::verify(tc1.root(), 1234, rows[1234], p) ? "true" : "false",
+                        tc1.tree().root() == tc1.root() ? "true" : "false");
+            auto st = tc1.authentication_structure({3, 77, 4000});
+            print_digests("table_commitment_auth_3_77_4000", st);
+            // XFE rows: hash_xfield_row flattening, 64 rows x 5 XFE columns
+            auto xv = splitmix(321, 64 * 15);
+            std::vector<std::vector<XFieldElement>> xrows(64, std::vector<XFieldElement>(5));
+            for (size_t r = 0; r < 64; r++)
+                for (size_t c = 0; c < 5; c++)
+                    xrows[r][c] = XFieldElement(xv[15 * r + 3 * c], xv[15 * r + 3 * c + 1], xv[15 * r + 3 * c + 2]);
+            TableCommitment tx = TableCommitment::commit_xfield_rows(xrows);
+            std::vector<Digest> xd(64);
+            for (size_t r = 0; r < 64; r++) xd[r] = hash_xfield_row(xrows[r]);
+            std::printf("{\"xfield_commitment_matches\": %s}\n",
+                        TableCommitment::from_digests(xd).root() == tx.root() ? "true" : "false");
+            print_digests("xfield_commitment_root", {tx.root()});
+            expect_throw<std::invalid_argument>("table_commitment_empty_throws",
+                                                [] { TableCommitment::from_digests({}); });
+        }
 
         // Whole commits (LDETable::compute_lde_and_merkle): golden shapes
         const size_t shapes[][3] = {{8, 7, 1}, {16, 9, 12}, {1024, 379, 1}};

@@ -1,6 +1,8 @@
-"""The drop-in C++ classes (include/triton_vm_amd/*.hpp, the reference's NTT / LDE /
-RandomizedLDE / LDETable / Tip5 / MerkleTree interfaces) on the GPU, checked against
-the golden fixtures written from the reference's own CPU code and the oracle.
+"""The drop-in C++ classes at the reference's include paths (ntt/ntt.hpp,
+lde/*.hpp, hash/tip5.hpp, merkle/merkle_tree.hpp, table/table_commitment.hpp,
+types/*.hpp: the NTT / LDE / RandomizedLDE / LDETable / Tip5 / MerkleTree /
+TableCommitment interfaces) on the GPU, checked against the golden fixtures
+written from the reference's own CPU code and the oracle.
 
 tests/native/dropin_test.cpp runs the classes and prints JSON lines; this test
 parses them.
@@ -75,3 +77,53 @@ def test_commit(out, golden):
     c = next(c for c in golden["commit"] if c["n"] == 8 and c["w"] == 7)
     assert out["two_step_commit_matches"] is True
     assert out["cw00"] == c["cw_00"]
+
+
+def test_tip5_tables_and_rounds(out, oracle):
+    assert out["tip5_lookup_table"] == [int(v) for v in oracle.lookup_table()]
+    assert out["tip5_mds_first_column"] == oracle.mds_first_column()
+    assert out["tip5_round_constants"] == [int(v) for v in oracle.round_constants()]
+    st = oracle.splitmix(77, 16)
+    want = [int(v) for v in st]
+    for r in range(5):
+        st = oracle.tip5_round(st, r)
+        want += [int(v) for v in st]
+    assert out["tip5_trace_77"] == want
+    assert out["tip5_rounds_match_trace"] is True
+    assert out["tip5_trace_leaves_permuted"] is True
+    assert out["tip5_round_range_throws"] is True
+
+
+def test_value_types(out):
+    assert out["field_inverses_ok"] is True
+    assert out["from_raw_u64_ok"] is True
+    assert out["hex_roundtrip"] is True
+    assert out["xfe_string"] == "(1, 2, 3)"
+    assert out["xfe_zero_inverse_throws"] is True
+
+
+def test_threaded_interpolate(out, oracle):
+    """NTT::interpolate from 8 host threads at once matches the oracle's INTT."""
+    assert out["threaded_interpolate_stable"] is True
+    for t in range(8):
+        col = oracle.splitmix(500 + t, 1 << (8 + t % 5))
+        assert out[f"threaded_interpolate_{t}"] == [int(v) for v in oracle.intt(col)], t
+
+
+def test_table_commitment(out, oracle, golden):
+    c = next(c for c in golden["commit"] if c["n"] == 1024 and c["w"] == 379)
+    assert out["table_commitment_1024_379"] == c["root"]
+    assert out["table_commitment_from_digests_matches"] is True
+    assert out["table_commitment_rows"] == 4096
+    assert out["table_commitment_verify"] is True
+    assert out["table_commitment_tree_root"] is True
+    t = oracle.table_rowmajor_draw(1, 1024, 379)
+    _, _, dig = oracle.commit(t, 1024, 379, 4096, want_digests=True)
+    nodes = oracle.merkle_heap(dig)
+    idx = oracle.auth_structure_indices(4096, [3, 77, 4000])
+    assert out["table_commitment_auth_3_77_4000"] == [int(v) for i in idx for v in nodes[i]]
+    xv = oracle.splitmix(321, 64 * 15).reshape(64, 15)
+    xd = np.stack([oracle.hash_varlen(xv[r]) for r in range(64)])
+    assert out["xfield_commitment_matches"] is True
+    assert out["xfield_commitment_root"] == [int(v) for v in oracle.merkle_heap(xd)[1]]
+    assert out["table_commitment_empty_throws"] is True

@@ -251,20 +251,24 @@ def test_commit_host_form(T, oracle, golden):
 
 
 def test_commit_full_size_sampled(T, oracle):
-    """2^20 x 379 -> 2^22: sampled columns/rows/paths against the oracle."""
+    """2^20 x 379 -> 2^22: every column at 8 sampled rows (the oracle's interpolant
+    evaluated at 7 w_M^j), columns 0 and 378 in full, sampled row digests and paths."""
     n, w, E = 1 << 20, 379, 4
     m = E * n
     t = oracle.table_rowmajor_draw(1, n, w)
     dt = T.dev(t)
-    x0, x378 = t[0:n].copy(), t[378 * n:379 * n].copy()
-    del t
     cw = T.empty(m * w)
     tree = T.empty(5 * (2 * m - 1))
     T.lib.commit(dt, w, n, None, 0, 1, 7, m, cw, tree, 0)
     T.sync()
-    assert np.array_equal(cw.download(m, 0), oracle.lde_column(x0, m))
-    assert np.array_equal(cw.download(m, 378 * m), oracle.lde_column(x378, m))
     rng = np.random.default_rng(3)
+    srows = [0, 1, 2, 3, m // 2 + 1, m - 1] + [int(r) for r in rng.integers(0, m, 2)]
+    want = oracle.lde_sample(t, n, w, m, srows)
+    for q, r in enumerate(srows):
+        assert np.array_equal(cw.gather_strided(r, m, w), want[:, q]), r
+    assert np.array_equal(cw.download(m, 0), oracle.lde_column(t[0:n], m))
+    assert np.array_equal(cw.download(m, 378 * m), oracle.lde_column(t[378 * n:379 * n], m))
+    del t
     rows = [0, 1, m - 1] + [int(r) for r in rng.integers(0, m, 40)]
     root = tree.download(5, 5 * (2 * m - 2))
     for j, r in enumerate(rows):

@@ -74,6 +74,33 @@ def test_lde_forms_agree_large(T, oracle, variant, logn, E, k, toff):
     assert np.array_equal(outs[0][m:2 * m], want)
 
 
+@pytest.mark.parametrize("logn", [17, 18, 19, 20, 21])
+def test_every_variant_full_size_against_oracle(T, oracle, tvhmod, variant, logn):
+    """Every variant the autotuner can pick, at the heights it picks them for
+    (2^17..2^21 rows, expansion 4): all columns at sampled rows against the
+    oracle's interpolant evaluated at 7 w_M^j."""
+    n, cols, E = 1 << logn, 4, 4
+    m = E * n
+    x = oracle.splitmix(900 + logn, n * cols)
+    dx = T.dev(x)
+    rows = [0, 1, 4097, m // 3, m - 2, m - 1]
+    want = oracle.lde_sample(x, n, cols, m, rows)
+    out = T.empty(m * cols)
+    ran = 0
+    for form in VARIANTS:
+        variant(logn, E, form)
+        try:
+            T.lib.lde_batch(dx, cols, n, None, 0, 1, 7, m, out)
+        except tvhmod.TVHError as e:
+            assert e.status == 7, e  # TVH_ERR_UNSUPPORTED: this variant cannot run the shape
+            continue
+        T.sync()
+        for q, r in enumerate(rows):
+            assert np.array_equal(out.gather_strided(r, m, cols), want[:, q]), (form, r)
+        ran += 1
+    assert ran >= 12
+
+
 def test_batch_plan_with_both_forms(T, oracle, variant):
     """A batch whose tables use different forms equals per-table commits."""
     shapes = [(10, 5, 4, WIDE), (12, 3, 4, SPLIT), (13, 7, 2, SPLIT | 4), (8, 11, 8, WIDE | 8),
@@ -117,3 +144,14 @@ def test_autotune_contract(T, variant):
     assert T.lib.lde_get_variant(14, 2) == chosen
     assert all(v > 0 for v in ms.values())
     assert ms[chosen] == min(ms.values())
+
+
+def test_autotune_skips_unsupported_variants(T, variant):
+    """At 2^21 rows and expansion 16 the 2^12-element row tiles cannot hold a
+    row's cosets: those variants are skipped (time -1), the rest are timed."""
+    chosen, ms = T.lib.lde_autotune(2, 1 << 21, 1 << 25)
+    variant(21, 16, chosen)
+    assert chosen in VARIANTS and ms[chosen] > 0
+    skipped = [v for v, t in ms.items() if t < 0]
+    assert skipped and all(v & QUARTER for v in skipped)
+    assert ms[chosen] == min(t for t in ms.values() if t > 0)

@@ -126,6 +126,25 @@ def test_lde_matches_horner(oracle, n, m, k, toff):
     assert [int(v) for v in got] == horner_lde(x, m, toff, 7, r)
 
 
+@pytest.mark.parametrize("n,m,k,toff", [(16, 64, 0, 1), (32, 128, 5, 3), (8, 8, 8, 1),
+                                         (64, 1024, 2, 11), (4, 64, 0, 7)])
+def test_lde_sample_matches_full_lde(oracle, n, m, k, toff):
+    """The sampled-row evaluation equals rows of the full (chunked) LDE and, with
+    truncate, of the XFE-rule LDE."""
+    cols = 3
+    x = oracle.splitmix(300 + n, n * cols)
+    r = oracle.splitmix(301 + k, k * cols).reshape(cols, k) if k else None
+    rows = [0, 1, m - 1, m // 2, 5 % m]
+    got = oracle.lde_sample(x, n, cols, m, rows, toff, 7, r)
+    full = oracle.lde_table(x, n, cols, m, toff, 7, r).reshape(cols, m)
+    assert np.array_equal(got, full[:, rows])
+    got_t = oracle.lde_sample(x, n, cols, m, rows, toff, 7, r, truncate=True)
+    for c in range(cols):
+        want = oracle.lde_limb_truncating(x[c * n:(c + 1) * n], m, toff, 7,
+                                          None if r is None else r[c])
+        assert np.array_equal(got_t[c], want[rows])
+
+
 def test_lde_contains_trace_at_stride(oracle):
     # offset-1 target: the LDE on <w_m> restricted to stride m/n is the trace
     x = oracle.splitmix(31, 64)
@@ -192,6 +211,20 @@ def test_tip5_golden(oracle, golden):
     for c in golden["hash_pair"]:
         got = oracle.hash_pair(oracle.splitmix(c["seed_l"], 5), oracle.splitmix(c["seed_r"], 5))
         assert [int(v) for v in got] == c["out"]
+
+
+def test_tip5_rounds(oracle, golden):
+    # the zero state is fixed by the S-box and the MDS, so round r maps it to
+    # the round-r constants (pins the constant index of each round)
+    rc = oracle.round_constants()
+    for r in range(5):
+        assert np.array_equal(oracle.tip5_round(np.zeros(16, np.uint64), r), rc[16 * r:16 * r + 16])
+    # rounds 0..4 in order compose to the golden permutations
+    for c in golden["tip5_permutation"]:
+        st = np.array(c["in"], np.uint64)
+        for r in range(5):
+            st = oracle.tip5_round(st, r)
+        assert [int(v) for v in st] == c["out"]
 
 
 def test_hash_pair_is_hash10_with_capacity_ones(oracle):
@@ -283,6 +316,11 @@ def test_against_reference_random(oracle, reference):
     for _ in range(20):
         st = oracle.splitmix(int(rng.integers(1, 1 << 30)), 16)
         assert np.array_equal(oracle.permutation(st), reference.permutation(st))
+        tr = reference.tip5_trace(st)  # Tip5::trace: state before and after each round
+        cur = st
+        for r in range(5):
+            cur = oracle.tip5_round(cur, r)
+            assert np.array_equal(cur, tr[r + 1])
     for n, m, k, toff in [(32, 128, 4, 1), (64, 64, 9, 3), (16, 256, 0, 11)]:
         x = oracle.splitmix(n + m, n)
         r = oracle.splitmix(k + 1, k) if k else None

@@ -145,13 +145,24 @@ const char* tvh_status_string(int s) {
 const char* tvh_last_error(void) { return g_err; }
 
 // ---------------------------------------------------------------- NTT
-int tvh_ntt_forward_batched_dev(uint64_t* d, size_t n, size_t cols, void* s) {
+// The batched NTT through the per-device arena (grown on first use of a larger size).
+static int ntt_batched_arena(uint64_t* d, size_t n, size_t cols, bool inv, hipStream_t st) {
     if (!d && cols) return set_error(TVH_ERR_INVALID_ARG, "null data");
-    return launch_ntt_batched(d, n, cols, false, STREAM(s));
+    if (cols == 0) return TVH_OK;
+    if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
+    DeviceState* ds = current_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    u64 need = ntt_scratch_elems(n, cols);
+    u64* scratch = (u64*)arena(ds, need * sizeof(u64), st);
+    if (!scratch) return TVH_ERR_OOM;
+    return launch_ntt_batched(d, n, cols, inv, scratch, need, st);
+}
+
+int tvh_ntt_forward_batched_dev(uint64_t* d, size_t n, size_t cols, void* s) {
+    return ntt_batched_arena(d, n, cols, false, STREAM(s));
 }
 int tvh_ntt_inverse_batched_dev(uint64_t* d, size_t n, size_t cols, void* s) {
-    if (!d && cols) return set_error(TVH_ERR_INVALID_ARG, "null data");
-    return launch_ntt_batched(d, n, cols, true, STREAM(s));
+    return ntt_batched_arena(d, n, cols, true, STREAM(s));
 }
 int tvh_ntt_forward_dev(uint64_t* d, size_t n, void* s) {
     return tvh_ntt_forward_batched_dev(d, n, 1, s);
@@ -199,6 +210,12 @@ int tvh_lde_batch_dev(const uint64_t* trace, size_t cols, size_t n, const uint64
     return launch_lde(trace, cols, n, bfe_rand(rand, k), toff, goff, m, false, out, STREAM(s));
 }
 
+int tvh_lde_prepare(size_t n, size_t m, uint64_t toff, uint64_t goff) {
+    return lde_prepare_consts(n, m, toff, goff);
+}
+
+int tvh_lde_release_constants(void) { return lde_release_consts(); }
+
 int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant) {
     return set_lde_variant(log_n, log_e, variant);
 }
@@ -241,10 +258,17 @@ int tvh_lde_autotune(size_t cols, size_t n, size_t m, int* chosen, float* ms) {
     float best = 0;
     int best_v = LDE_WIDE;
     int prev = lde_variant(logN, logE);
+    if (!rc) rc = lde_prepare_consts(n, m, 1, 7);
+    bool any = false;
     for (int i = 0; i < kNumLdeVariants && !rc; i++) {
         set_lde_variant(logN, logE, kLdeVariants[i]);
         RandSpec none;
         rc = launch_lde(trace, tc, n, none, 1, 7, m, false, out, st);  // warm-up
+        if (rc == TVH_ERR_UNSUPPORTED) {  // this variant cannot run the shape: skip it
+            rc = TVH_OK;
+            if (ms) ms[i] = -1.0f;
+            continue;
+        }
         if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = set_error(TVH_ERR_DEVICE, "event");
         for (int r = 0; r < 3 && !rc; r++) rc = launch_lde(trace, tc, n, none, 1, 7, m, false, out, st);
         if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
@@ -253,11 +277,13 @@ int tvh_lde_autotune(size_t cols, size_t n, size_t m, int* chosen, float* ms) {
         if (!rc && hipEventElapsedTime(&t, e0, e1) != hipSuccess) rc = set_error(TVH_ERR_DEVICE, "elapsed");
         t /= 3;
         if (ms) ms[i] = t;
-        if (!rc && (i == 0 || t < best)) {
+        if (!rc && (!any || t < best)) {
             best = t;
             best_v = kLdeVariants[i];
+            any = true;
         }
     }
+    if (!rc && !any) rc = set_error(TVH_ERR_UNSUPPORTED, "no LDE variant runs this shape");
     set_lde_variant(logN, logE, rc ? prev : best_v);
     if (chosen) *chosen = rc ? prev : best_v;
     if (st) (void)hipStreamSynchronize(st);
@@ -283,6 +309,19 @@ int tvh_lde_xfe_limbs_dev(const uint64_t* limbs, size_t xcols, size_t n, const u
                       STREAM(s));
 }
 
+// Interleaved XFE columns through a limb-major staging copy in `work`
+// (3 * xcols * (n + m) u64).
+static int lde_xfe_interleaved(const uint64_t* xt, size_t xcols, size_t n, const uint64_t* rand,
+                               size_t k, uint64_t toff, uint64_t goff, size_t m, uint64_t* xout,
+                               int xfe_r, u64* work, hipStream_t st) {
+    u64* in_lm = work;
+    u64* out_lm = work + 3 * xcols * n;
+    int rc = launch_xfe_interleave(xt, in_lm, xcols, n, true, st);
+    if (!rc) rc = tvh_lde_xfe_limbs_dev(in_lm, xcols, n, rand, k, toff, goff, m, out_lm, xfe_r, st);
+    if (!rc) rc = launch_xfe_interleave(out_lm, xout, xcols, m, false, st);
+    return rc;
+}
+
 int tvh_lde_xfe_batch_dev(const uint64_t* xt, size_t xcols, size_t n, const uint64_t* rand,
                           size_t k, uint64_t toff, uint64_t goff, size_t m, uint64_t* xout,
                           int xfe_r, void* s) {
@@ -293,12 +332,7 @@ int tvh_lde_xfe_batch_dev(const uint64_t* xt, size_t xcols, size_t n, const uint
     if (!ds) return TVH_ERR_DEVICE;
     u64* work = (u64*)arena(ds, 3 * xcols * (n + m) * sizeof(u64), STREAM(s));
     if (!work) return TVH_ERR_OOM;
-    u64* in_lm = work;
-    u64* out_lm = work + 3 * xcols * n;
-    int rc = launch_xfe_interleave(xt, in_lm, xcols, n, true, STREAM(s));
-    if (!rc) rc = tvh_lde_xfe_limbs_dev(in_lm, xcols, n, rand, k, toff, goff, m, out_lm, xfe_r, s);
-    if (!rc) rc = launch_xfe_interleave(out_lm, xout, xcols, m, false, STREAM(s));
-    return rc;
+    return lde_xfe_interleaved(xt, xcols, n, rand, k, toff, goff, m, xout, xfe_r, work, STREAM(s));
 }
 
 // ---------------------------------------------------------------- Tip5
@@ -314,6 +348,16 @@ int tvh_hash_xfield_rows_dev(const uint64_t* t, size_t rows, size_t xcols, uint6
 int tvh_tip5_permutation_batch_dev(uint64_t* st, size_t num, void* s) {
     if (num && !st) return set_error(TVH_ERR_INVALID_ARG, "null states");
     return launch_permutation(st, num, STREAM(s));
+}
+
+int tvh_tip5_rounds_dev(uint64_t* st, size_t num, uint32_t first, uint32_t count, uint64_t* trace,
+                        void* s) {
+    if (num && !st) return set_error(TVH_ERR_INVALID_ARG, "null states");
+    if (first > (uint32_t)tip5::ROUNDS || count > (uint32_t)tip5::ROUNDS - first)
+        return set_error(TVH_ERR_RANGE, "rounds [%u, %u + %u) beyond round %d", first, first, count,
+                         tip5::ROUNDS - 1);
+    if (!current_device()) return TVH_ERR_DEVICE;
+    return launch_permutation_rounds(st, num, first, count, trace, STREAM(s));
 }
 
 int tvh_hash_pairs_dev(const uint64_t* l, const uint64_t* r, uint64_t* o, size_t count, void* s) {
@@ -555,10 +599,12 @@ static int ntt_host(uint64_t* data, size_t n, bool inv) {
     if (rc) return rc;
     HostCall hc;
     if (hc.rc) return hc.rc;
+    // a scratch of its own: concurrent host calls share nothing on the device
     u64* d = hc.alloc(n);
-    if (!d) return hc.rc;
+    u64* scratch = hc.alloc(n);
+    if (hc.rc) return hc.rc;
     if ((rc = hc.up(d, data, n))) return rc;
-    if ((rc = launch_ntt_batched(d, n, 1, inv, hc.st))) return rc;
+    if ((rc = launch_ntt_batched(d, n, 1, inv, scratch, n, hc.st))) return rc;
     if ((rc = hc.down(data, d, n))) return rc;
     return hc.sync();
 }
@@ -597,13 +643,15 @@ int tvh_lde_xfe_batch_host(const uint64_t* xt, size_t cols, size_t n, const uint
     if (rc) return rc;
     HostCall hc;
     if (hc.rc) return hc.rc;
+    if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
     u64* dt = hc.alloc(3 * cols * n);
     u64* dr = hc.alloc(rlen);
     u64* dout = hc.alloc(3 * cols * m);
+    u64* work = hc.alloc(3 * cols * (n + m));  // private staging: no shared arena
     if (hc.rc) return hc.rc;
     if ((rc = hc.up(dt, xt, 3 * cols * n)) || (rc = hc.up(dr, rand, rlen))) return rc;
-    if ((rc = tvh_lde_xfe_batch_dev(dt, cols, n, k ? dr : nullptr, k, toff, goff, m, dout, xfe_r,
-                                    hc.st)))
+    if ((rc = lde_xfe_interleaved(dt, cols, n, k ? dr : nullptr, k, toff, goff, m, dout, xfe_r,
+                                  work, hc.st)))
         return rc;
     if ((rc = hc.down(xout, dout, 3 * cols * m))) return rc;
     return hc.sync();
@@ -620,6 +668,26 @@ int tvh_tip5_permutation_batch_host(uint64_t* states, size_t num) {
     if ((rc = hc.up(d, states, num * 16))) return rc;
     if ((rc = launch_permutation(d, num, hc.st))) return rc;
     if ((rc = hc.down(states, d, num * 16))) return rc;
+    return hc.sync();
+}
+
+int tvh_tip5_rounds_host(uint64_t* states, size_t num, uint32_t first, uint32_t count,
+                         uint64_t* trace) {
+    if (!num) return TVH_OK;
+    if (!states) return set_error(TVH_ERR_INVALID_ARG, "null states");
+    if (first > (uint32_t)tip5::ROUNDS || count > (uint32_t)tip5::ROUNDS - first)
+        return set_error(TVH_ERR_RANGE, "rounds [%u, %u + %u) beyond round %d", first, first, count,
+                         tip5::ROUNDS - 1);
+    HostCall hc;
+    if (hc.rc) return hc.rc;
+    u64* d = hc.alloc(num * 16);
+    u64* dt = trace ? hc.alloc(num * 16 * (count + 1)) : nullptr;
+    if (hc.rc) return hc.rc;
+    int rc;
+    if ((rc = hc.up(d, states, num * 16))) return rc;
+    if ((rc = launch_permutation_rounds(d, num, first, count, dt, hc.st))) return rc;
+    if ((rc = hc.down(states, d, num * 16))) return rc;
+    if (trace && (rc = hc.down(trace, dt, num * 16 * (count + 1)))) return rc;
     return hc.sync();
 }
 

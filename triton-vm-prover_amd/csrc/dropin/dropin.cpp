@@ -1,18 +1,22 @@
 // Drop-in implementations of the reference's NTT / LDE / RandomizedLDE /
-// LDETable / Tip5 / MerkleTree classes over the C ABI of libtvh.  Each method
-// forwards its bulk work to one tvh_*_host call; the host side only converts
-// between the reference's containers and flat column-major u64 buffers.
+// LDETable / Tip5 / MerkleTree / TableCommitment classes over the C ABI of
+// libtvh.  Each method forwards its bulk work to one tvh_*_host call; the host
+// side only converts between the reference's containers and flat column-major
+// u64 buffers.
+#include <algorithm>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <unordered_set>
-#include <algorithm>
 
-#include "triton_vm_amd/lde.hpp"
-#include "triton_vm_amd/merkle.hpp"
-#include "triton_vm_amd/ntt.hpp"
-#include "triton_vm_amd/tip5.hpp"
+#include "hash/tip5.hpp"
+#include "lde/lde_randomized.hpp"
+#include "lde/lde_table.hpp"
+#include "merkle/merkle_tree.hpp"
+#include "ntt/ntt.hpp"
+#include "table/table_commitment.hpp"
+#include "tip5_constants.hpp"
 #include "tvh.h"
 
 namespace triton_vm {
@@ -137,8 +141,12 @@ std::vector<std::vector<BFieldElement>> LDE::extend_table(
     const std::vector<std::vector<BFieldElement>>& table, const ArithmeticDomain& trace_domain,
     const ArithmeticDomain& quotient_domain) {
     if (table.empty()) return {};
+    // (only the domains' members are read here: the prover's own
+    // ArithmeticDomain methods are defined in its master_table.cpp)
+    ArithmeticDomain td = trace_domain;
+    td.offset = BFieldElement::one();
     return LDETable::extend_table_with_randomizers(
-        table, trace_domain.with_offset(BFieldElement::one()), quotient_domain,
+        table, td, quotient_domain,
         std::vector<std::vector<BFieldElement>>(table[0].size()));
 }
 
@@ -162,7 +170,8 @@ BPolynomial RandomizedLDE::mul_zerofier_with(const ArithmeticDomain& d, const BP
 }
 
 bool RandomizedLDE::verify_zerofier(const ArithmeticDomain& d, const BPolynomial& z) {
-    for (const auto& x : d.values())
+    BFieldElement x = d.offset;
+    for (size_t i = 0; i < d.length; i++, x *= d.generator)
         if (!z.evaluate(x).is_zero()) return false;
     return true;
 }
@@ -261,6 +270,44 @@ Digest LDETable::compute_lde_and_merkle(const std::vector<std::vector<BFieldElem
 }
 
 // ------------------------------------------------------------------ Tip5
+namespace {
+std::array<uint8_t, 256> make_lookup_table() {
+    std::array<uint8_t, 256> t{};
+    for (unsigned x = 0; x < 256; x++) {
+        unsigned y = x + 1;
+        t[x] = (uint8_t)(((y * y % 257) * y % 257) - 1);
+    }
+    return t;
+}
+std::array<int64_t, Tip5::STATE_SIZE> make_mds_column() {
+    std::array<int64_t, Tip5::STATE_SIZE> c{};
+    for (size_t i = 0; i < Tip5::STATE_SIZE; i++) c[i] = tip5c::kMdsFirstColumn[i];
+    return c;
+}
+std::array<BFieldElement, Tip5::NUM_ROUNDS * Tip5::STATE_SIZE> make_round_constants() {
+    std::array<BFieldElement, Tip5::NUM_ROUNDS * Tip5::STATE_SIZE> c{};
+    for (size_t i = 0; i < c.size(); i++) c[i] = BFieldElement(tip5c::kRoundConstants[i]);
+    return c;
+}
+}  // namespace
+
+const std::array<uint8_t, 256> Tip5::LOOKUP_TABLE = make_lookup_table();
+const std::array<int64_t, Tip5::STATE_SIZE> Tip5::MDS_MATRIX_FIRST_COLUMN = make_mds_column();
+const std::array<BFieldElement, Tip5::NUM_ROUNDS * Tip5::STATE_SIZE> Tip5::ROUND_CONSTANTS =
+    make_round_constants();
+
+Tip5::Tip5() { state.fill(BFieldElement::zero()); }
+
+Tip5 Tip5::init() { return Tip5(); }
+
+void Tip5::round(size_t round_index) {
+    if (round_index >= NUM_ROUNDS) throw std::out_of_range("Tip5 round index out of range");
+    amd::ensure_device();
+    amd::check(tvh_tip5_rounds_host(reinterpret_cast<uint64_t*>(state.data()), 1,
+                                    (uint32_t)round_index, 1, nullptr),
+               "Tip5::round");
+}
+
 void Tip5::permutation() {
     amd::ensure_device();
     amd::check(tvh_tip5_permutation_batch_host(reinterpret_cast<uint64_t*>(state.data()), 1),
@@ -276,9 +323,12 @@ void Tip5::permutation_batch(std::vector<std::array<BFieldElement, STATE_SIZE>>&
 }
 
 std::vector<std::array<BFieldElement, Tip5::STATE_SIZE>> Tip5::trace() {
-    // The device library runs whole permutations; per-round states (used by the
-    // AIR's hash table, not by the commitment) are not exposed.
-    throw std::runtime_error("Tip5::trace: per-round states are not provided by the GPU library");
+    std::vector<std::array<BFieldElement, STATE_SIZE>> tr(NUM_ROUNDS + 1);
+    amd::ensure_device();
+    amd::check(tvh_tip5_rounds_host(reinterpret_cast<uint64_t*>(state.data()), 1, 0,
+                                    (uint32_t)NUM_ROUNDS, reinterpret_cast<uint64_t*>(tr.data())),
+               "Tip5::trace");
+    return tr;
 }
 
 void Tip5::absorb(const std::vector<BFieldElement>& elements) {
@@ -352,6 +402,20 @@ MerkleTree::MerkleTree(const std::vector<Digest>& leaves) {
                "MerkleTree");
     nodes_[0] = Digest();
 }
+
+MerkleTree MerkleTree::parallel_new(const std::vector<Digest>& leaves) { return MerkleTree(leaves); }
+
+MerkleTree MerkleTree::from_heap_nodes(std::vector<Digest> nodes) {
+    size_t n = nodes.size() / 2;
+    if (nodes.size() % 2 || !pow2(n)) throw std::invalid_argument("heap nodes: size must be 2n, n a power of 2");
+    MerkleTree t;
+    t.nodes_ = std::move(nodes);
+    t.nodes_[0] = Digest();
+    t.num_leaves_ = n;
+    return t;
+}
+
+Digest MerkleTree::root() const { return nodes_[1]; }
 
 size_t MerkleTree::height() const {
     size_t h = 0;
@@ -432,6 +496,85 @@ bool MerkleTree::verify_authentication_structure(const Digest& root, size_t n,
     }
     auto it = known.find(1);
     return it != known.end() && it->second == root;
+}
+
+// ------------------------------------------------------------------ TableCommitment
+TableCommitment::TableCommitment(std::unique_ptr<MerkleTree> tree) : tree_(std::move(tree)) {}
+
+TableCommitment TableCommitment::from_digests(const std::vector<Digest>& row_digests) {
+    return TableCommitment(std::make_unique<MerkleTree>(row_digests));
+}
+
+namespace {
+// Row digests of a column-major table, then the heap-layout tree, in one GPU
+// round trip (tvh_hash_bfield_rows_host, tvh_merkle_tree_host).
+std::unique_ptr<MerkleTree> commit_colmajor(const std::vector<uint64_t>& cm, size_t rows,
+                                            size_t width) {
+    if (rows == 0) throw std::invalid_argument("Cannot create Merkle tree with no leaves");
+    if (!pow2(rows)) throw std::invalid_argument("Number of leaves must be a power of 2");
+    amd::ensure_device();
+    std::vector<Digest> dig(rows);
+    uint64_t dummy = 0;
+    amd::check(tvh_hash_bfield_rows_host(width ? cm.data() : &dummy, rows, width,
+                                         reinterpret_cast<uint64_t*>(dig.data())),
+               "TableCommitment::commit");
+    std::vector<Digest> nodes(2 * rows);
+    amd::check(tvh_merkle_tree_host(reinterpret_cast<const uint64_t*>(dig.data()), rows,
+                                    TVH_TREE_HEAP, reinterpret_cast<uint64_t*>(nodes.data()), nullptr),
+               "TableCommitment::commit");
+    return std::make_unique<MerkleTree>(MerkleTree::from_heap_nodes(std::move(nodes)));
+}
+}  // namespace
+
+TableCommitment TableCommitment::commit_bfield_rows(const BFieldElement* const* rows, size_t n,
+                                                    size_t w) {
+    std::vector<uint64_t> cm(n * w);
+    for (size_t r = 0; r < n; r++)
+        for (size_t c = 0; c < w; c++) cm[c * n + r] = rows[r][c].value();
+    return TableCommitment(commit_colmajor(cm, n, w));
+}
+
+TableCommitment TableCommitment::commit_xfield_rows(const XFieldElement* const* rows, size_t n,
+                                                    size_t w) {
+    // flattened (c0, c1, c2) per element: hash_xfield_row's order
+    std::vector<uint64_t> cm(n * 3 * w);
+    for (size_t r = 0; r < n; r++)
+        for (size_t c = 0; c < w; c++)
+            for (size_t k = 0; k < 3; k++) cm[(3 * c + k) * n + r] = rows[r][c].coeff(k).value();
+    return TableCommitment(commit_colmajor(cm, n, 3 * w));
+}
+
+TableCommitment TableCommitment::commit_bfield_rows(const std::vector<std::vector<BFieldElement>>& rows) {
+    size_t w = rows.empty() ? 0 : rows[0].size();
+    std::vector<const BFieldElement*> p(rows.size());
+    for (size_t i = 0; i < rows.size(); i++) {
+        if (rows[i].size() != w) throw std::invalid_argument("TableCommitment: ragged table");
+        p[i] = rows[i].data();
+    }
+    return commit_bfield_rows(p.data(), p.size(), w);
+}
+
+TableCommitment TableCommitment::commit_xfield_rows(const std::vector<std::vector<XFieldElement>>& rows) {
+    size_t w = rows.empty() ? 0 : rows[0].size();
+    std::vector<const XFieldElement*> p(rows.size());
+    for (size_t i = 0; i < rows.size(); i++) {
+        if (rows[i].size() != w) throw std::invalid_argument("TableCommitment: ragged table");
+        p[i] = rows[i].data();
+    }
+    return commit_xfield_rows(p.data(), p.size(), w);
+}
+
+Digest TableCommitment::root() const { return tree_->root(); }
+size_t TableCommitment::num_rows() const { return tree_->num_leaves(); }
+std::vector<Digest> TableCommitment::authentication_path(size_t row_index) const {
+    return tree_->authentication_path(row_index);
+}
+std::vector<Digest> TableCommitment::authentication_structure(const std::vector<size_t>& idx) const {
+    return tree_->authentication_structure(idx);
+}
+bool TableCommitment::verify(const Digest& root, size_t row_index, const Digest& row_digest,
+                             const std::vector<Digest>& auth_path) {
+    return MerkleTree::verify(root, row_index, row_digest, auth_path);
 }
 
 Digest hash_row(const std::vector<BFieldElement>& row) {

@@ -946,7 +946,13 @@ struct ConstVal {
 };
 static std::mutex g_const_mu;
 static std::map<ConstKey, ConstVal> g_consts;
+// At most this many tables (each <= E*(N1+N2)*8 <= 1 MB) are kept; the cache
+// never evicts, since kernels in flight may read any entry.
+constexpr size_t kMaxConstEntries = 256;
 
+// The coset tables of one (shape, split, offsets) key: found in the cache, or
+// computed now (one allocation and a wait on a private stream; tvh_lde_prepare
+// does this ahead of time).
 static int lde_consts(DeviceState* ds, u32 logN, u32 a, u32 E, u64 toff, u64 goff, ConstVal* out) {
     ConstKey key{logN, a, E, toff, goff, ds->device};
     std::lock_guard<std::mutex> lk(g_const_mu);
@@ -955,6 +961,11 @@ static int lde_consts(DeviceState* ds, u32 logN, u32 a, u32 E, u64 toff, u64 gof
         *out = it->second;
         return TVH_OK;
     }
+    if (g_consts.size() >= kMaxConstEntries)
+        return set_error(TVH_ERR_RANGE,
+                         "more than %zu distinct LDE shape/offset tables; release them with "
+                         "tvh_lde_release_constants",
+                         kMaxConstEntries);
     u32 b = logN - a;
     u64 n1 = 1ull << a, n2 = 1ull << b;
     u64 M = (u64)E << logN;
@@ -992,6 +1003,42 @@ static int lde_consts(DeviceState* ds, u32 logN, u32 a, u32 E, u64 toff, u64 gof
     ConstVal v{mem, mem + E * n1};
     g_consts[key] = v;
     *out = v;
+    return TVH_OK;
+}
+
+int lde_prepare_consts(u64 n, u64 m, u64 toff, u64 goff) {
+    if (!is_pow2(n) || !is_pow2(m)) return set_error(TVH_ERR_NOT_POW2, "lde lengths");
+    if (m < n) return set_error(TVH_ERR_INVALID_ARG, "target_len < trace_len");
+    if (m / n > 16) return set_error(TVH_ERR_UNSUPPORTED, "expansion > 16");
+    if (toff == 0 || toff >= gl::P || goff >= gl::P)
+        return set_error(TVH_ERR_INVALID_ARG, "offsets must be canonical, trace offset nonzero");
+    u32 logN = log2u(n);
+    if (logN > 23) return set_error(TVH_ERR_UNSUPPORTED, "trace_len > 2^23");
+    DeviceState* ds = current_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    u32 E = (u32)(m / n);
+    // every row tile a kernel variant can use (2^12..2^14 elements) gives its own split
+    for (u32 lt = LOG_TILE - 2; lt <= LOG_TILE; lt++) {
+        u32 a = choose_split(logN, E, lt);
+        if (a > 12) continue;
+        ConstVal cv;
+        int rc = lde_consts(ds, logN, a, E, toff, goff, &cv);
+        if (rc) return rc;
+    }
+    return TVH_OK;
+}
+
+int lde_release_consts() {
+    std::lock_guard<std::mutex> lk(g_const_mu);
+    if (hipDeviceSynchronize() != hipSuccess) return set_error(TVH_ERR_DEVICE, "device sync");
+    for (auto& kv : g_consts) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipSetDevice(kv.first.dev);
+        (void)hipFree(kv.second.pa);
+        (void)hipSetDevice(dev);
+    }
+    g_consts.clear();
     return TVH_OK;
 }
 
@@ -1189,7 +1236,17 @@ int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg
     return check_launch();
 }
 
-int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st) {
+u64 ntt_scratch_elems(u64 n, u64 cols) {
+    // columns go through the scratch in groups of at most 256 MB
+    u64 group = cols;
+    u64 max_group = (256ull << 20) / (n * sizeof(u64));
+    if (max_group < 1) max_group = 1;
+    if (group > max_group) group = max_group;
+    return group * n;
+}
+
+int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, u64* scratch, u64 scratch_elems,
+                       hipStream_t st) {
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
     u32 logN = log2u(n);
@@ -1201,13 +1258,12 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st)
         b = 12;
         a = logN - b;
     }
-    // columns processed in groups so that the scratch stays <= 256 MB
-    u64 group = cols;
-    u64 max_group = (256ull << 20) / (n * sizeof(u64));
-    if (max_group < 1) max_group = 1;
-    if (group > max_group) group = max_group;
-    u64* scratch = (u64*)arena(ds, group * n * sizeof(u64), st);
-    if (!scratch) return TVH_ERR_OOM;
+    // step 1 writes each group of columns to the scratch, step 2 writes it back
+    u64 group = scratch_elems / n;
+    if (!scratch || group == 0)
+        return set_error(TVH_ERR_INVALID_ARG, "NTT scratch of %llu elements < n = %llu",
+                         (unsigned long long)scratch_elems, (unsigned long long)n);
+    if (group > cols) group = cols;
     u64 ninv = gl::inv(n % gl::P);
     int rc;
     for (u64 c0 = 0; c0 < cols; c0 += group) {

@@ -84,6 +84,9 @@ struct tvh_server {
     std::unordered_map<uint64_t, Job> jobs;  // element references survive inserts
     std::vector<hipStream_t> slots;
     std::vector<int64_t> slot_job;
+    u64* root_stage = nullptr;  // pinned host memory: 5 u64 per slot, written by the slot stream
+    size_t waiters = 0;         // threads inside tvh_server_wait (destroy waits for them)
+    std::condition_variable waiters_cv;
     tvh::FirstFitArena arena;
     uint8_t* base = nullptr;
     bool stop = false;
@@ -108,8 +111,10 @@ struct tvh_server {
         done_cv.notify_all();
     }
 
-    // Enqueue an admitted job on its slot stream.
-    int launch(Job& j, hipStream_t st) {
+    // Enqueue an admitted job on its slot stream: inputs, commit, then the
+    // root into the slot's pinned staging words, then the completion event.
+    int launch(Job& j, int slot) {
+        hipStream_t st = slots[slot];
         const tvh_job_desc& d = j.d;
         u64* tr = at(j, j.lay.trace);
         u64* rd = at(j, j.lay.rand);
@@ -133,6 +138,9 @@ struct tvh_server {
             rc = tvh_commit_dev(tr, d.num_cols, d.trace_len, rn ? rd : nullptr, d.rand_len,
                                 d.trace_offset, d.target_offset, d.target_len,
                                 at(j, j.lay.cw), at(j, j.lay.tree), TVH_TREE_LEVEL_ORDER, st);
+        if (!rc && hipMemcpyAsync(root_stage + 5 * slot, at(j, j.lay.tree) + 5 * (2 * d.target_len - 2),
+                                  40, hipMemcpyDeviceToHost, st) != hipSuccess)
+            rc = set_error(TVH_ERR_DEVICE, "root download");
         if (!rc && hipEventRecord(j.ev, st) != hipSuccess) rc = set_error(TVH_ERR_DEVICE, "event record");
         return rc;
     }
@@ -151,9 +159,8 @@ struct tvh_server {
                 if (q != hipSuccess)
                     rc = set_error(TVH_ERR_DEVICE, "job %lld: %s", (long long)slot_job[s],
                                    hipGetErrorString(q));
-                else if (hipMemcpy(j.root, at(j, j.lay.tree) + 5 * (2 * j.d.target_len - 2), 40,
-                                   hipMemcpyDeviceToHost) != hipSuccess)
-                    rc = set_error(TVH_ERR_DEVICE, "root download");
+                else  // the event follows the root's copy into this slot's staging words
+                    std::memcpy(j.root, root_stage + 5 * s, 40);
                 if (rc) arena.release(j.off, j.lay.total);
                 finish(j, rc, rc ? tvh_last_error() : nullptr);
                 slot_job[s] = -1;
@@ -180,7 +187,7 @@ struct tvh_server {
                 j.off = off;
                 j.t.admit_s = now_s();
                 j.state = TVH_JOB_RUNNING;
-                int rc = launch(j, slots[slot]);
+                int rc = launch(j, slot);
                 if (rc) {
                     std::string msg = tvh_last_error();
                     // work already enqueued on the slot may still use the range
@@ -280,12 +287,14 @@ int tvh_server_create(const tvh_server_config* cfg, tvh_server** out) {
     s->arena.reset(budget);
     s->slots.assign(nslots, nullptr);
     s->slot_job.assign(nslots, -1);
-    bool ok = hipStreamCreateWithFlags(&s->qstream, hipStreamNonBlocking) == hipSuccess;
+    bool ok = hipHostMalloc((void**)&s->root_stage, 40 * (size_t)nslots, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&s->qstream, hipStreamNonBlocking) == hipSuccess;
     for (auto& st : s->slots) ok = ok && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
         for (auto st : s->slots)
             if (st) (void)hipStreamDestroy(st);
         if (s->qstream) (void)hipStreamDestroy(s->qstream);
+        if (s->root_stage) (void)hipHostFree(s->root_stage);
         (void)hipFree(s->base);
         delete s;
         return set_error(TVH_ERR_DEVICE, "stream creation");
@@ -301,6 +310,12 @@ int tvh_server_submit(tvh_server* s, const tvh_job_desc* d, uint64_t* job_id) {
         return set_error(TVH_ERR_NOT_POW2, "trace_len and target_len must be powers of 2");
     if (d->target_len < d->trace_len || d->rand_len > d->trace_len)
         return set_error(TVH_ERR_INVALID_ARG, "need rand_len <= trace_len <= target_len");
+    // the job's coset tables are built here, outside the scheduler's lock, so
+    // that admission finds them cached (no allocation, no wait)
+    if (d->num_cols) {
+        int rc = tvh_lde_prepare(d->trace_len, d->target_len, d->trace_offset, d->target_offset);
+        if (rc) return rc;
+    }
     hipEvent_t ev;
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
         return set_error(TVH_ERR_DEVICE, "event");
@@ -333,14 +348,27 @@ int tvh_server_poll(tvh_server* s, uint64_t id, int* state) {
 int tvh_server_wait(tvh_server* s, uint64_t id, uint64_t* root, tvh_job_times* times) {
     if (!s) return set_error(TVH_ERR_INVALID_ARG, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(id);
-    if (it == s->jobs.end()) return set_error(TVH_ERR_RANGE, "unknown job %llu", (unsigned long long)id);
-    Job* j = &it->second;
-    s->done_cv.wait(lk, [&] { return j->state >= TVH_JOB_DONE; });
-    if (times) *times = j->t;
-    if (j->state == TVH_JOB_FAILED) return set_error(j->status, "%s", j->err.c_str());
-    if (root) std::memcpy(root, j->root, 40);
-    return TVH_OK;
+    if (s->jobs.find(id) == s->jobs.end())
+        return set_error(TVH_ERR_RANGE, "unknown job %llu", (unsigned long long)id);
+    // the job is looked up again after every wake-up: a concurrent release may
+    // have erased it; destroy waits until no thread is left in here
+    s->waiters++;
+    Job* j = nullptr;
+    s->done_cv.wait(lk, [&] {
+        auto it = s->jobs.find(id);
+        j = it == s->jobs.end() ? nullptr : &it->second;
+        return !j || j->state >= TVH_JOB_DONE;
+    });
+    int rc = TVH_OK;
+    if (!j) {
+        rc = set_error(TVH_ERR_RANGE, "job %llu was released while waiting", (unsigned long long)id);
+    } else {
+        if (times) *times = j->t;
+        if (j->state == TVH_JOB_FAILED) rc = set_error(j->status, "%s", j->err.c_str());
+        else if (root) std::memcpy(root, j->root, 40);
+    }
+    if (--s->waiters == 0) s->waiters_cv.notify_all();
+    return rc;
 }
 
 int tvh_server_open(tvh_server* s, uint64_t id, const size_t* leaves, size_t count,
@@ -423,6 +451,7 @@ int tvh_server_release(tvh_server* s, uint64_t id) {
     (void)hipEventDestroy(j.ev);
     s->jobs.erase(it);
     s->cv.notify_all();
+    s->done_cv.notify_all();  // waiters on this id return TVH_ERR_RANGE
     return TVH_OK;
 }
 
@@ -461,13 +490,26 @@ int tvh_server_destroy(tvh_server* s) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
-    for (auto& kv : s->jobs)
-        if (kv.second.state == TVH_JOB_RUNNING) kv.second.state = TVH_JOB_FAILED;
+    {
+        // jobs still marked running never had their completion observed: fail
+        // them under the lock, wake their waiters and wait until every waiter
+        // has left tvh_server_wait before the condition variables go away
+        std::unique_lock<std::mutex> lk(s->mu);
+        for (auto& kv : s->jobs)
+            if (kv.second.state == TVH_JOB_RUNNING) {
+                kv.second.state = TVH_JOB_FAILED;
+                kv.second.status = TVH_ERR_DEVICE;
+                kv.second.err = "server destroyed while the job was running";
+            }
+        s->done_cv.notify_all();
+        s->waiters_cv.wait(lk, [&] { return s->waiters == 0; });
+    }
     (void)hipStreamSynchronize(s->qstream);
     (void)hipStreamDestroy(s->qstream);
     for (auto& kv : s->jobs)
         if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
     if (s->qbuf) (void)hipFree(s->qbuf);
+    if (s->root_stage) (void)hipHostFree(s->root_stage);
     (void)hipFree(s->base);
     delete s;
     return TVH_OK;

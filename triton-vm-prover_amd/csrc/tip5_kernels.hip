@@ -152,6 +152,36 @@ __global__ void __launch_bounds__(256) k_permutation(u64* __restrict__ st, u64 n
     for (int k = 0; k < tip5::STATE; k++) st[i * tip5::STATE + k] = tip5::from_m(s[k]);
 }
 
+// Rounds [first, first + count) of the permutation on num states (in place).
+// trace (may be null) receives, per state, the input and the state after each
+// of those rounds: trace[(i * (count + 1) + j) * 16 + k].  Outputs canonical.
+__global__ void __launch_bounds__(256) k_permutation_rounds(u64* __restrict__ st, u64 num,
+                                                            u32 first, u32 count,
+                                                            u64* __restrict__ trace) {
+    __shared__ uint8_t lut[256];
+    tip5::load_lut(lut);
+    __syncthreads();
+    u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num) return;
+    u64 s[tip5::STATE];
+#pragma unroll
+    for (int k = 0; k < tip5::STATE; k++) s[k] = tip5::to_m(st[i * tip5::STATE + k]);
+    u64* tr = trace ? trace + i * (count + 1) * tip5::STATE : nullptr;
+    if (tr) {
+#pragma unroll
+        for (int k = 0; k < tip5::STATE; k++) tr[k] = st[i * tip5::STATE + k];
+    }
+    for (u32 r = 0; r < count; r++) {
+        tip5::round_m(s, (int)(first + r), lut);
+        if (tr) {
+#pragma unroll
+            for (int k = 0; k < tip5::STATE; k++) tr[(r + 1) * tip5::STATE + k] = tip5::from_m(s[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < tip5::STATE; k++) st[i * tip5::STATE + k] = tip5::from_m(s[k]);
+}
+
 __device__ __forceinline__ void hash_pair(const u64* l, const u64* r, u64 s[tip5::STATE],
                                           const uint8_t* lut) {
 #pragma unroll
@@ -356,6 +386,14 @@ int launch_permutation(u64* states, u64 num, hipStream_t st) {
     if (num == 0) return TVH_OK;
     hipLaunchKernelGGL(k_permutation, dim3((u32)((num + 255) / 256)), dim3(256), 0, st, states,
                        num);
+    return check_launch();
+}
+
+int launch_permutation_rounds(u64* states, u64 num, u32 first, u32 count, u64* trace,
+                              hipStream_t st) {
+    if (num == 0) return TVH_OK;
+    hipLaunchKernelGGL(k_permutation_rounds, dim3((u32)((num + 255) / 256)), dim3(256), 0, st,
+                       states, num, first, count, trace);
     return check_launch();
 }
 

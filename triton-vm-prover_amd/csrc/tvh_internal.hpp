@@ -41,6 +41,7 @@ struct DeviceState {
     size_t arena_bytes = 0;
 };
 
+
 // State of the current device, initialised on first use (thread-safe).
 // Returns nullptr and sets the error on failure.
 DeviceState* current_device();
@@ -61,7 +62,14 @@ inline u32 log2u(u64 x) {
 }
 
 // ---- kernels launchers (ntt_kernels.hip) ----
-int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, hipStream_t st);
+// Batched NTT in place through a scratch of scratch_elems u64 (at least n;
+// columns go through it in groups of scratch_elems / n).  Launches that share
+// a scratch must be ordered (one stream).
+int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, u64* scratch, u64 scratch_elems,
+                       hipStream_t st);
+// The scratch launch_ntt_batched uses for (n, cols) from the per-device arena
+// (groups of at most 256 MB).
+u64 ntt_scratch_elems(u64 n, u64 cols);
 // LDE of `cols` B-field columns (column-major in, column-major out).  rand may
 // be null (k = 0).  truncate selects the XFE evaluation rule.  lift_stride:
 // randomizer of column c is rand[(c / lift_div) * rand_col_stride + ...] with
@@ -152,6 +160,11 @@ struct LdePlan {
 // (kBatchColLt, kBatchRowLt; the wide form keeps its full tile).
 int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
                 u64 m, bool truncate, u64* out, LdePlan* plan, bool batch);
+// Builds (once) the coset tables every LDE variant of this shape and these
+// offsets reads, so that later LDE launches find them cached.
+int lde_prepare_consts(u64 n, u64 m, u64 toff, u64 goff);
+// Waits for the device and frees every cached coset table.
+int lde_release_consts();
 // Phases of the multi-table LDE, in launch order.
 enum LdePhase {
     LDE_PHASE_STEP1 = 0,   // k_step1_multi<true>
@@ -193,6 +206,8 @@ int launch_merkle_multi(const MerkleArgs* d_args, const u32* d_start, u32 nseg, 
                         hipStream_t st);
 int launch_hash_rows(const u64* tab, u64 rows, u64 cols, u64* dig, hipStream_t st);
 int launch_permutation(u64* states, u64 num, hipStream_t st);
+int launch_permutation_rounds(u64* states, u64 num, u32 first, u32 count, u64* trace,
+                              hipStream_t st);
 // mode 0 absorb (pad and absorb all), 1 squeeze, 2 init
 int launch_sponge(u64* state, const u64* data, u64 len, u64* out, int mode, hipStream_t st);
 int launch_xfe_leaves(const u64* xfe, u64 n, u64* leaves, hipStream_t st);

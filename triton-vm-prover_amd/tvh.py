@@ -98,6 +98,10 @@ _SIGS = {
     "tvh_hash_bfield_rows_dev": [_vp, _sz, _sz, _vp, _vp],
     "tvh_hash_xfield_rows_dev": [_vp, _sz, _sz, _vp, _vp],
     "tvh_tip5_permutation_batch_dev": [_vp, _sz, _vp],
+    "tvh_tip5_rounds_dev": [_vp, _sz, C.c_uint32, C.c_uint32, _vp, _vp],
+    "tvh_tip5_rounds_host": [_vp, _sz, C.c_uint32, C.c_uint32, _vp],
+    "tvh_lde_prepare": [_sz, _sz, _u64, _u64],
+    "tvh_lde_release_constants": [],
     "tvh_hash_pairs_dev": [_vp, _vp, _vp, _sz, _vp],
     "tvh_merkle_tree_dev": [_vp, _vp, _sz, _i32, _vp],
     "tvh_merkle_root_dev": [_vp, _vp, _sz, _vp],
@@ -212,6 +216,9 @@ class TVH:
         self._call("tvh_lde_xfe_limbs_dev", _ptr(limbs), xcols, n, _ptr(rand), k, toff, goff, m,
                    _ptr(out), int(xfe_rand), self.stream_ptr(stream))
 
+    def lde_prepare(self, n, m, toff=1, goff=7):
+        self._call("tvh_lde_prepare", n, m, toff, goff)
+
     # -- Tip5 -------------------------------------------------------------------
     def hash_bfield_rows(self, table, rows, cols, digests, stream=None):
         self._call("tvh_hash_bfield_rows_dev", _ptr(table), rows, cols, _ptr(digests),
@@ -223,6 +230,10 @@ class TVH:
 
     def permutation_batch(self, states, num, stream=None):
         self._call("tvh_tip5_permutation_batch_dev", _ptr(states), num, self.stream_ptr(stream))
+
+    def tip5_rounds(self, states, num, first, count, trace=None, stream=None):
+        self._call("tvh_tip5_rounds_dev", _ptr(states), num, first, count, _ptr(trace),
+                   self.stream_ptr(stream))
 
     def hash_pairs(self, left, right, out, count, stream=None):
         self._call("tvh_hash_pairs_dev", _ptr(left), _ptr(right), _ptr(out), count,
