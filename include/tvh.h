@@ -190,6 +190,51 @@ int tvh_lde_variant_count(void);
 int tvh_lde_autotune(size_t num_cols, size_t trace_len, size_t target_len, int* chosen,
                      float* ms);
 
+/* ---- two-phase coset evaluation (the frugal commit's primitives) ---------- */
+
+/* Phase 1: d_coefficients[c*n + i] = INTT_n(trace_c)_i * trace_offset^-i, the
+ * trace's interpolant in coefficient form.  d_coefficients may equal d_trace
+ * (in place).  Uses the per-device arena for the NTT's transpose step.
+ * Replaces compute_trace_coefficients_gpu (randomized_lde_kernel.cuh:127-134). */
+int tvh_trace_coefficients_dev(const uint64_t* d_trace, size_t num_cols, size_t trace_len,
+                               uint64_t trace_offset, uint64_t* d_coefficients, void* stream);
+
+/* Phase 2: d_output[c*n + j] = P_c(coset_offset * w_n^j), j < n = trace_len,
+ * with P_c = sum_i coeff_i X^i + (X^n - trace_offset^n) * r_c(X), r_c =
+ * d_rand[c*rand_len + i] (rand_len <= trace_len; d_rand may be NULL when 0).
+ * Called once per coset g_k = fri_offset * w_M^k of an M-point domain, row j
+ * of coset k is row j*(M/n) + k of the M-point codeword.  d_tail_scratch is
+ * accepted for the reference's signature and not used (may be NULL).  Uses the
+ * per-device arena for the NTT's transpose step.
+ * Replaces evaluate_coset_from_coefficients_gpu (randomized_lde_kernel.cuh:152-163). */
+int tvh_evaluate_coset_dev(const uint64_t* d_coefficients, size_t num_cols, size_t trace_len,
+                           const uint64_t* d_rand, size_t rand_len, uint64_t trace_offset,
+                           uint64_t coset_offset, uint64_t* d_output, uint64_t* d_tail_scratch,
+                           void* stream);
+
+/* d_leaves[(j*num_cosets + coset_idx)*5 + e] = d_coset_digests[j*5 + e],
+ * j < coset_len: a coset's row digests into the full domain's leaf order.
+ * Replaces qzc_scatter_digests_strided_kernel (gpu_stark.cu:993-1007). */
+int tvh_scatter_digests_strided_dev(const uint64_t* d_coset_digests, uint64_t* d_leaves,
+                                    size_t coset_len, size_t coset_idx, size_t num_cosets,
+                                    void* stream);
+
+/* ---- layout transforms ------------------------------------------------------ */
+
+/* Row-major n x width (element (r, c) at r*width + c) to column-major (at
+ * c*n + r), and back; not in place.  The XFE forms take n x xwidth XFE
+ * elements, limbs interleaved row-major ((r*xwidth + c)*3 + k) to limb-major
+ * columns ((3c + k)*n + r), and back.  Replace qzc_rowmajor_to_colmajor_{bfe,xfe}
+ * and qzc_colmajor_to_rowmajor_{bfe,xfe} (gpu_stark.cu:125-214). */
+int tvh_rowmajor_to_colmajor_bfe_dev(const uint64_t* d_rowmajor, uint64_t* d_colmajor, size_t n,
+                                     size_t width, void* stream);
+int tvh_rowmajor_to_colmajor_xfe_dev(const uint64_t* d_rowmajor, uint64_t* d_colmajor, size_t n,
+                                     size_t xwidth, void* stream);
+int tvh_colmajor_to_rowmajor_bfe_dev(const uint64_t* d_colmajor, uint64_t* d_rowmajor, size_t n,
+                                     size_t width, void* stream);
+int tvh_colmajor_to_rowmajor_xfe_dev(const uint64_t* d_colmajor, uint64_t* d_rowmajor, size_t n,
+                                     size_t xwidth, void* stream);
+
 /* ---- Tip5 (src/gpu/kernels/row_hash_kernel.cu, tip5_kernel.cu) ------------- */
 
 /* digests[r] = Tip5::hash_varlen(row r) over a column-major table.
@@ -322,6 +367,14 @@ int tvh_commit_batch_dev(const tvh_table_desc* tables, size_t count, void* strea
  * counterpart): d[i] = SplitMix64(seed + (i+1) * golden gamma) reduced below p
  * (values >= p are replaced by v - p). */
 int tvh_fill_uniform_dev(uint64_t* d, size_t count, uint64_t seed, void* stream);
+
+/* Synthetic seeded table (no reference counterpart): the column-major
+ * rows x cols table whose row-major draw order (element (r, c) is draw
+ * r*cols + c) is the SplitMix64 stream with state `seed`, draws >= p rejected
+ * -- the same table as the oracle's seeded draw, so a timed table can be
+ * reproduced on the host.  Uses 65 u64 of the per-device arena (calls that
+ * use the arena must be ordered on one stream). */
+int tvh_fill_table_dev(uint64_t* d, size_t rows, size_t cols, uint64_t seed, void* stream);
 
 /* Streaming ("frugal") commit: the same result as tvh_commit_dev without
  * materialising the codewords.  Columns are extended chunk_cols at a time into

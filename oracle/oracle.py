@@ -279,6 +279,7 @@ class Reference:
             "ref_auth_structure": (C.c_long, [_u64p, sz, _szp, sz, _u64p, sz]),
             "ref_commit": (i32, [_u64p, sz, sz, u64, C.c_void_p, sz, u64, sz,
                                  C.c_void_p, C.c_void_p, _u64p]),
+            "ref_commit_parallel": (i32, [_u64p, sz, sz, u64, sz, u64, _u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -376,3 +377,10 @@ class Reference:
                                  None if dg is None else dg.ctypes.data, root)
         assert rc == 0
         return root, cw, (None if dg is None else dg.reshape(m, 5))
+
+    def commit_parallel(self, trace_cm, n, cols, m, trace_off=1, target_off=7):
+        """Root of the reference commit with OpenMP over columns and rows (no randomizers)."""
+        t = np.ascontiguousarray(trace_cm, dtype=np.uint64).reshape(-1)
+        root = np.zeros(5, np.uint64)
+        assert self.lib.ref_commit_parallel(t, n, cols, trace_off, m, target_off, root) == 0
+        return root

@@ -211,4 +211,39 @@ int ref_commit(const uint64_t* trace, size_t n, size_t cols, uint64_t toff, cons
     } catch (const std::exception&) { return -1; }
 }
 
+// The same commit with the work spread over the host's cores: the reference's
+// per-column LDE (RandomizedLDE::extend_column_with_randomizer, lde_randomized.cpp:71-154)
+// in an OpenMP loop over columns, Tip5::hash_varlen over rows in an OpenMP
+// loop, then MerkleTree (whose levels are OpenMP loops, merkle_tree.cpp:41-61):
+// the parallel form main_gpu.cpp:113-123 sets its OpenMP threads up for.
+int ref_commit_parallel(const uint64_t* trace, size_t n, size_t cols, uint64_t toff, size_t m,
+                        uint64_t goff, uint64_t* root) {
+    try {
+        std::vector<std::vector<BFieldElement>> lde(cols);
+        const ArithmeticDomain td = domain(n, toff), qd = domain(m, goff);
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (long c = 0; c < (long)cols; c++) {
+            try {
+                lde[c] = RandomizedLDE::extend_column_with_randomizer(to_bfe(trace + c * n, n), td, qd,
+                                                                       {});
+            } catch (const std::exception&) {
+#pragma omp atomic write
+                bad = 1;
+            }
+        }
+        if (bad) return -1;
+        std::vector<Digest> leaves(m);
+#pragma omp parallel for schedule(static)
+        for (long r = 0; r < (long)m; r++) {
+            std::vector<BFieldElement> row(cols);
+            for (size_t c = 0; c < cols; c++) row[c] = lde[c][r];
+            leaves[r] = Tip5::hash_varlen(row);
+        }
+        MerkleTree t(leaves);
+        from_digest(t.root(), root);
+        return 0;
+    } catch (const std::exception&) { return -1; }
+}
+
 }  // extern "C"

@@ -361,3 +361,15 @@ def test_commit_frugal_heap_layout(T, oracle):
     T.lib.commit_frugal(T.dev(t), w, n, None, 0, 1, 7, m, 8, work, tree, 1)
     root, _, _ = oracle.commit(t, n, w, m)
     assert np.array_equal(T.host(tree).reshape(-1, 5)[1], root)
+
+
+# ------------------------------------------------------------------ seeded tables
+@pytest.mark.parametrize("seed,rows,cols", [(1, 8, 7), (5, 1, 1), (12, 16, 9), (2, 1000, 37),
+                                            (1, 1 << 20, 379)])
+def test_fill_table_is_the_oracle_draw(T, oracle, seed, rows, cols):
+    """tvh_fill_table_dev equals the oracle's row-major SplitMix64 draw with
+    rejection; at 2^20 x 379 from seed 1 the stream rejects draw 265931911, so
+    every element after it is shifted by one draw."""
+    d = T.empty(rows * cols)
+    T.lib.fill_table(d, rows, cols, seed)
+    assert np.array_equal(T.host(d, rows * cols), oracle.table_rowmajor_draw(seed, rows, cols))

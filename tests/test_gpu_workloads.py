@@ -115,6 +115,12 @@ def test_serve_2000_jobs(T, tvhmod, oracle):
     for j in jobs:  # the first job of each height keeps its buffers for the opening checks
         keep.setdefault(j["n"], j["i"])
     keep_ids = set(keep.values())
+    # the plain path's buffers (largest shape) before the server takes its budget
+    nmax = max(j["n"] for j in jobs)
+    w = jobs[0]["w"]
+    bufs = (hip_rt.DeviceBuffer(nmax * w), hip_rt.DeviceBuffer(E * nmax * w),
+            hip_rt.DeviceBuffer(5 * (2 * E * nmax - 1)))
+    free0, _ = hip_rt.mem_info()
     srv = lib.server_create(max_inflight=4)
     ids = [None] * len(jobs)
     ready = [threading.Event() for _ in jobs]
@@ -155,17 +161,17 @@ def test_serve_2000_jobs(T, tvhmod, oracle):
             e.set()
         th.join()
         lib.server_destroy(srv)
+    free1, _ = hip_rt.mem_info()
+    print(f"serve: free HBM before create {free0 / 2**30:.1f} GiB, after destroy "
+          f"{free1 / 2**30:.1f} GiB", flush=True)
     if err:
         raise err[0]
+    assert free1 > free0 - (4 << 30), "the server's arena was not returned on destroy"
     lat = np.array([r[2] - r[0] for r in res])
     print(f"serve: commit latency p50 {np.percentile(lat, 50):.3f} s, "
           f"p99 {np.percentile(lat, 99):.3f} s", flush=True)
 
-    # the plain path for every job, on buffers of the largest shape
-    nmax = max(j["n"] for j in jobs)
-    w = jobs[0]["w"]
-    bufs = (hip_rt.DeviceBuffer(nmax * w), hip_rt.DeviceBuffer(E * nmax * w),
-            hip_rt.DeviceBuffer(5 * (2 * E * nmax - 1)))
+    # the plain path for every job, on the buffers of the largest shape
     stream = hip_rt.Stream()
     for k, j in enumerate(jobs):
         root = _plain_commit(lib, hip_rt, j, bufs, stream)

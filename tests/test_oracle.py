@@ -330,3 +330,15 @@ def test_against_reference_random(oracle, reference):
     a = oracle.commit(t, 32, 23, 128, want_digests=True)
     b = reference.commit(t, 32, 23, 128, want_digests=True)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2])
+
+
+def test_reference_parallel_commit(oracle, reference):
+    """The parallel reference commit (the bench's CPU baseline) gives the same root
+    as the reference's serial commit and the oracle."""
+    t = oracle.table_rowmajor_draw(4, 64, 37)
+    reference.set_threads(4)
+    par = reference.commit_parallel(t, 64, 37, 256)
+    reference.set_threads(1)
+    ser = reference.commit(t, 64, 37, 256)[0]
+    assert np.array_equal(par, ser)
+    assert np.array_equal(par, oracle.commit(t, 64, 37, 256)[0])

@@ -335,6 +335,68 @@ int tvh_lde_xfe_batch_dev(const uint64_t* xt, size_t xcols, size_t n, const uint
     return lde_xfe_interleaved(xt, xcols, n, rand, k, toff, goff, m, xout, xfe_r, work, STREAM(s));
 }
 
+// ---------------------------------------------------------------- two-phase coset evaluation
+int tvh_trace_coefficients_dev(const uint64_t* trace, size_t cols, size_t n, uint64_t toff,
+                               uint64_t* coeffs, void* s) {
+    if (cols && (!trace || !coeffs)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (!cols) return TVH_OK;
+    if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "trace_len must be a power of 2");
+    if (toff == 0 || toff >= TVH_P) return set_error(TVH_ERR_INVALID_ARG, "trace offset must be canonical and nonzero");
+    hipStream_t st = STREAM(s);
+    if (coeffs != trace &&
+        hipMemcpyAsync(coeffs, trace, 8 * n * cols, hipMemcpyDeviceToDevice, st) != hipSuccess)
+        return set_error(TVH_ERR_DEVICE, "coefficient copy");
+    int rc = ntt_batched_arena(coeffs, n, cols, true, st);
+    if (rc || toff == 1) return rc;
+    return launch_scale_powers(coeffs, coeffs, n, cols, gl::inv(toff), nullptr, 0, 0, st);
+}
+
+int tvh_evaluate_coset_dev(const uint64_t* coeffs, size_t cols, size_t n, const uint64_t* rand,
+                           size_t k, uint64_t toff, uint64_t coset_offset, uint64_t* out,
+                           uint64_t* tail_scratch, void* s) {
+    (void)tail_scratch;  // one NTT covers the randomizer tail (g^n is constant on the coset)
+    if (cols && (!coeffs || !out)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (k && !rand) return set_error(TVH_ERR_INVALID_ARG, "null randomizers");
+    if (!cols) return TVH_OK;
+    if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "trace_len must be a power of 2");
+    if (k > n) return set_error(TVH_ERR_UNSUPPORTED, "randomizer_len > trace_len");
+    if (toff >= TVH_P || coset_offset >= TVH_P) return set_error(TVH_ERR_INVALID_ARG, "offsets must be canonical");
+    hipStream_t st = STREAM(s);
+    u64 delta = gl::sub(gl::pow(coset_offset, n), gl::pow(toff, n));
+    int rc = launch_scale_powers(coeffs, out, n, cols, coset_offset, k ? rand : nullptr, k, delta, st);
+    if (rc) return rc;
+    return ntt_batched_arena(out, n, cols, false, st);
+}
+
+int tvh_scatter_digests_strided_dev(const uint64_t* coset_digests, uint64_t* leaves,
+                                    size_t coset_len, size_t coset_idx, size_t num_cosets, void* s) {
+    if (coset_len && (!coset_digests || !leaves)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (coset_idx >= num_cosets) return set_error(TVH_ERR_RANGE, "coset %zu of %zu", coset_idx, num_cosets);
+    return launch_scatter_digests(coset_digests, leaves, coset_len, coset_idx, num_cosets, STREAM(s));
+}
+
+// ---------------------------------------------------------------- layout transforms
+int tvh_rowmajor_to_colmajor_bfe_dev(const uint64_t* rm, uint64_t* cm, size_t n, size_t width,
+                                     void* s) {
+    if (n && width && (!rm || !cm)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (rm == cm && n > 1 && width > 1) return set_error(TVH_ERR_INVALID_ARG, "transpose is not in place");
+    return launch_transpose(rm, cm, n, width, STREAM(s));
+}
+int tvh_rowmajor_to_colmajor_xfe_dev(const uint64_t* rm, uint64_t* cm, size_t n, size_t xwidth,
+                                     void* s) {
+    return tvh_rowmajor_to_colmajor_bfe_dev(rm, cm, n, 3 * xwidth, s);
+}
+int tvh_colmajor_to_rowmajor_bfe_dev(const uint64_t* cm, uint64_t* rm, size_t n, size_t width,
+                                     void* s) {
+    if (n && width && (!rm || !cm)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    if (rm == cm && n > 1 && width > 1) return set_error(TVH_ERR_INVALID_ARG, "transpose is not in place");
+    return launch_transpose(cm, rm, width, n, STREAM(s));
+}
+int tvh_colmajor_to_rowmajor_xfe_dev(const uint64_t* cm, uint64_t* rm, size_t n, size_t xwidth,
+                                     void* s) {
+    return tvh_colmajor_to_rowmajor_bfe_dev(cm, rm, n, 3 * xwidth, s);
+}
+
 // ---------------------------------------------------------------- Tip5
 int tvh_hash_bfield_rows_dev(const uint64_t* t, size_t rows, size_t cols, uint64_t* dg, void* s) {
     if (rows && (!dg || (cols && !t))) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
@@ -502,6 +564,16 @@ int tvh_fill_uniform_dev(uint64_t* d, size_t count, uint64_t seed, void* s) {
     if (count && !d) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
     if (!current_device()) return TVH_ERR_DEVICE;
     return launch_fill_uniform(d, count, seed, STREAM(s));
+}
+
+int tvh_fill_table_dev(uint64_t* d, size_t rows, size_t cols, uint64_t seed, void* s) {
+    if (rows && cols && !d) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
+    DeviceState* ds = current_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    if (!rows || !cols) return TVH_OK;
+    u64* rej = (u64*)arena(ds, fill_table_scratch_elems() * sizeof(u64), STREAM(s));
+    if (!rej) return TVH_ERR_OOM;
+    return launch_fill_table(d, rows, cols, seed, rej, STREAM(s));
 }
 
 size_t tvh_commit_frugal_workspace_elems(size_t cols, size_t n, size_t m, size_t chunk) {

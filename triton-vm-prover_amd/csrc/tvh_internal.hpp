@@ -182,6 +182,20 @@ int launch_fill_uniform(u64* d, u64 count, u64 seed, hipStream_t st);
 int launch_xfe_interleave(const u64* src, u64* dst, u64 cols, u64 len, bool to_limb_major,
                           hipStream_t st);
 
+// ---- coset evaluation, digest scatter, transposes (coset_kernels.hip) ----
+// y[c*n + i] = (x[c*n + i] + [i < k] rand[c*k + i] * delta) * base^i (x may be y)
+int launch_scale_powers(const u64* x, u64* y, u64 n, u64 cols, u64 base, const u64* rand, u64 k,
+                        u64 delta, hipStream_t st);
+// leaves[(j * num_cosets + coset) * 5 + e] = dig[j * 5 + e], j < len
+int launch_scatter_digests(const u64* dig, u64* leaves, u64 len, u64 coset, u64 num_cosets,
+                           hipStream_t st);
+// dst[c * rows + r] = src[r * cols + c]
+int launch_transpose(const u64* src, u64* dst, u64 rows, u64 cols, hipStream_t st);
+// d (column-major rows x cols) = the SplitMix64 stream from seed in row-major
+// draw order, draws >= p rejected; rej: fill_table_scratch_elems() u64 of scratch
+int launch_fill_table(u64* d, u64 rows, u64 cols, u64 seed, u64* rej, hipStream_t st);
+u64 fill_table_scratch_elems();
+
 // ---- Tip5 launchers (tip5_kernels.hip) ----
 struct HashArgs {
     const u64* tab;

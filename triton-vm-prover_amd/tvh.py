@@ -95,6 +95,13 @@ _SIGS = {
     "tvh_lde_column_dev": [_vp, _sz, _vp, _sz, _u64, _u64, _sz, _vp, _vp],
     "tvh_lde_xfe_batch_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp, _i32, _vp],
     "tvh_lde_xfe_limbs_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp, _i32, _vp],
+    "tvh_trace_coefficients_dev": [_vp, _sz, _sz, _u64, _vp, _vp],
+    "tvh_evaluate_coset_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _vp, _vp, _vp],
+    "tvh_scatter_digests_strided_dev": [_vp, _vp, _sz, _sz, _sz, _vp],
+    "tvh_rowmajor_to_colmajor_bfe_dev": [_vp, _vp, _sz, _sz, _vp],
+    "tvh_rowmajor_to_colmajor_xfe_dev": [_vp, _vp, _sz, _sz, _vp],
+    "tvh_colmajor_to_rowmajor_bfe_dev": [_vp, _vp, _sz, _sz, _vp],
+    "tvh_colmajor_to_rowmajor_xfe_dev": [_vp, _vp, _sz, _sz, _vp],
     "tvh_hash_bfield_rows_dev": [_vp, _sz, _sz, _vp, _vp],
     "tvh_hash_xfield_rows_dev": [_vp, _sz, _sz, _vp, _vp],
     "tvh_tip5_permutation_batch_dev": [_vp, _sz, _vp],
@@ -120,6 +127,7 @@ _SIGS = {
     "tvh_batch_plan_destroy": [_vp],
     "tvh_commit_batch_dev": [_vp, _sz, _vp],
     "tvh_fill_uniform_dev": [_vp, _sz, _u64, _vp],
+    "tvh_fill_table_dev": [_vp, _sz, _sz, _u64, _vp],
     "tvh_commit_frugal_workspace_elems": [_sz, _sz, _sz, _sz],
     "tvh_commit_frugal_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _sz, _vp, _vp, _i32, _vp],
     "tvh_lde_set_variant": [C.c_uint32, C.c_uint32, _i32],
@@ -218,6 +226,24 @@ class TVH:
 
     def lde_prepare(self, n, m, toff=1, goff=7):
         self._call("tvh_lde_prepare", n, m, toff, goff)
+
+    # -- two-phase coset evaluation, digest scatter, transposes ---------------------
+    def trace_coefficients(self, trace, cols, n, toff, coeffs, stream=None):
+        self._call("tvh_trace_coefficients_dev", _ptr(trace), cols, n, toff, _ptr(coeffs),
+                   self.stream_ptr(stream))
+
+    def evaluate_coset(self, coeffs, cols, n, rand, k, toff, coset_offset, out, stream=None):
+        self._call("tvh_evaluate_coset_dev", _ptr(coeffs), cols, n, _ptr(rand), k, toff,
+                   coset_offset, _ptr(out), None, self.stream_ptr(stream))
+
+    def scatter_digests(self, dig, leaves, length, coset, num_cosets, stream=None):
+        self._call("tvh_scatter_digests_strided_dev", _ptr(dig), _ptr(leaves), length, coset,
+                   num_cosets, self.stream_ptr(stream))
+
+    def transpose(self, src, dst, n, width, to_colmajor=True, xfe=False, stream=None):
+        name = ("tvh_rowmajor_to_colmajor_" if to_colmajor else "tvh_colmajor_to_rowmajor_") + \
+               ("xfe_dev" if xfe else "bfe_dev")
+        self._call(name, _ptr(src), _ptr(dst), n, width, self.stream_ptr(stream))
 
     # -- Tip5 -------------------------------------------------------------------
     def hash_bfield_rows(self, table, rows, cols, digests, stream=None):
@@ -324,6 +350,9 @@ class TVH:
 
     def fill_uniform(self, d, count, seed, stream=None):
         self._call("tvh_fill_uniform_dev", _ptr(d), count, seed, self.stream_ptr(stream))
+
+    def fill_table(self, d, rows, cols, seed, stream=None):
+        self._call("tvh_fill_table_dev", _ptr(d), rows, cols, seed, self.stream_ptr(stream))
 
     def commit_frugal(self, trace, cols, n, rand, k, toff, goff, m, chunk, work, tree,
                       layout=TREE_LEVEL_ORDER, stream=None):
