@@ -462,7 +462,8 @@ int tvh_server_buffers(tvh_server* server, uint64_t job_id, const uint64_t** d_c
 int tvh_server_release(tvh_server* server, uint64_t job_id);
 int tvh_server_stats(tvh_server* server, size_t* arena_used, size_t* arena_size,
                      size_t* queued, size_t* running);
-/* Waits for running jobs, fails queued ones, frees everything. */
+/* Waits for running jobs, fails queued ones, frees everything; waits for the
+ * device so that the arena is back in the free pool when it returns. */
 int tvh_server_destroy(tvh_server* server);
 
 /* ---- host-pointer forms (synchronous; CudaBackend-style) ------------------ */

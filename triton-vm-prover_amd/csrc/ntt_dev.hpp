@@ -657,10 +657,26 @@ struct ExpandStore {
         if (R * Ns == (1 << b)) {  // base = j: u1 = j + Ns*r
             f = gl::mont_mul(f, twiddle(rt, 0, (u32)(k1 * j), A.logN));
             const u64 step = twiddle(rt, 0, (u32)(k1 * Ns), A.logN);
+            // factors f * step^r as C interleaved chains step^C apart: a
+            // dependency depth of about R / C products instead of R - 1
+            constexpr int C = R >= 4 ? 4 : R;
+            u64 fr[C];
+            fr[0] = f;
+            u64 stride = step;
+            if constexpr (C == 4) {
+                const u64 step2 = gl::mont_sqr(step);
+                fr[1] = gl::mont_mul(f, step);
+                fr[2] = gl::mont_mul(f, step2);
+                fr[3] = gl::mont_mul(fr[1], step2);
+                stride = gl::mont_sqr(step2);
+            } else if constexpr (C == 2) {
+                fr[1] = gl::mont_mul(f, step);
+                stride = gl::mont_sqr(step);
+            }
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                dst[o + E * (u32)(j + r * Ns)] = gl::mont_mul(v[r], f);
-                if (r + 1 < R) f = gl::mont_mul(f, step);
+                dst[o + E * (u32)(j + r * Ns)] = gl::mont_mul(v[r], fr[r % C]);
+                if (r + C < R) fr[r % C] = gl::mont_mul(fr[r % C], stride);
             }
         } else {
 #pragma unroll

@@ -187,15 +187,20 @@ struct tvh_server {
                 j.off = off;
                 j.t.admit_s = now_s();
                 j.state = TVH_JOB_RUNNING;
+                slot_job[slot] = (int64_t)id;  // the slot is taken while the lock is released
+                // enqueue without the lock: a host-memory upload may take a while,
+                // and waits, polls and opening queries of other jobs need the lock
+                // (the job's map entry stays put: only finished jobs are erased)
+                lk.unlock();
                 int rc = launch(j, slot);
+                lk.lock();
                 if (rc) {
                     std::string msg = tvh_last_error();
                     // work already enqueued on the slot may still use the range
                     (void)hipStreamSynchronize(slots[slot]);
                     arena.release(j.off, j.lay.total);
                     finish(j, rc, msg.c_str());
-                } else {
-                    slot_job[slot] = (int64_t)id;
+                    slot_job[slot] = -1;
                 }
                 progress = true;
             }
@@ -271,6 +276,9 @@ int tvh_server_create(const tvh_server_config* cfg, tvh_server** out) {
     size_t budget = cfg ? cfg->hbm_budget_bytes : 0;
     int nslots = cfg && cfg->max_inflight > 0 ? cfg->max_inflight : 4;
     if (budget == 0) {
+        // memory freed just before may be released lazily: wait for the device
+        // so that the free figure includes it
+        if (hipDeviceSynchronize() != hipSuccess) return set_error(TVH_ERR_DEVICE, "device sync");
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return set_error(TVH_ERR_DEVICE, "hipMemGetInfo");
         budget = fr > (32ull << 30) ? fr - (16ull << 30) : fr / 2;
@@ -508,11 +516,33 @@ int tvh_server_destroy(tvh_server* s) {
     (void)hipStreamDestroy(s->qstream);
     for (auto& kv : s->jobs)
         if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
-    if (s->qbuf) (void)hipFree(s->qbuf);
-    if (s->root_stage) (void)hipHostFree(s->root_stage);
-    (void)hipFree(s->base);
+    int rc = TVH_OK;
+    hipError_t e;
+    if (s->qbuf && (e = hipFree(s->qbuf)) != hipSuccess)
+        rc = set_error(TVH_ERR_DEVICE, "query buffer free: %s", hipGetErrorString(e));
+    if (s->root_stage && (e = hipHostFree(s->root_stage)) != hipSuccess && !rc)
+        rc = set_error(TVH_ERR_DEVICE, "root staging free: %s", hipGetErrorString(e));
+    // The runtime returns a large allocation to the device's free pool some
+    // time after hipFree (measured: hundreds of GB reappear later, not at the
+    // call).  Wait, bounded, until the free figure has grown by the arena's
+    // size, so that a caller can reuse the memory as soon as this returns.
+    size_t free0 = 0, total = 0, arena_bytes = s->arena.size();
+    (void)hipDeviceSynchronize();
+    (void)hipMemGetInfo(&free0, &total);
+    if ((e = hipFree(s->base)) != hipSuccess && !rc)
+        rc = set_error(TVH_ERR_DEVICE, "arena free: %s", hipGetErrorString(e));
+    if (!rc) {
+        const size_t want = free0 + arena_bytes - arena_bytes / 64;  // within ~1.5 %
+        auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            size_t fr = 0;
+            if (hipMemGetInfo(&fr, &total) != hipSuccess || fr >= want) break;
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(20)) break;
+            std::this_thread::sleep_for(std::chrono::milliseconds(2));
+        }
+    }
     delete s;
-    return TVH_OK;
+    return rc;
 }
 
 }  // extern "C"

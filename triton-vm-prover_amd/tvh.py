@@ -157,9 +157,11 @@ _SIGS = {
 
 
 def _ptr(x):
-    """Address of a DeviceBuffer / numpy array / None."""
+    """Address of a DeviceBuffer / numpy array / raw address (int) / None."""
     if x is None:
         return None
+    if isinstance(x, int):
+        return x
     if hasattr(x, "data_ptr"):
         return x.data_ptr()
     return x.ctypes.data
