@@ -86,6 +86,19 @@ int tvh_ntt_forward_batched_dev(uint64_t* d_data, size_t n, size_t num_cols, voi
  * Replaces ntt_inverse_batched_gpu (ntt_kernel.cuh:102-107). */
 int tvh_ntt_inverse_batched_dev(uint64_t* d_data, size_t n, size_t num_cols, void* stream);
 
+/* tvh_ntt_forward_batched_dev (inverse == 0) or tvh_ntt_inverse_batched_dev
+ * (inverse != 0) without any allocation: the step between the two transform
+ * halves goes through a static 32 MiB device scratch per GPU (held until the
+ * process ends) instead of the growable arena, so n <= 2^22
+ * (TVH_ERR_UNSUPPORTED beyond).  tvh_init must have been called for the
+ * current device (TVH_ERR_DEVICE otherwise).  Calls that overlap on one GPU
+ * (two streams, two threads) race on that scratch; order them on one stream.
+ * No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+int tvh_ntt_batched_noalloc_dev(uint64_t* d_data, size_t n, size_t num_cols, int inverse,
+                                void* stream);
+
 /* Single-column forms. Replace ntt_forward_gpu / ntt_inverse_gpu
  * (ntt_kernel.cuh:37-58). */
 int tvh_ntt_forward_dev(uint64_t* d_data, size_t n, void* stream);

@@ -85,9 +85,24 @@ DeviceState* current_device() {
     if (ds.device < 0) {
         if (build_tables(ds)) return nullptr;
         if (ntt_set_attributes()) return nullptr;
+        if (noalloc_scratch_setup(&ds)) return nullptr;
         ds.device = dev;
     }
     return &ds;
+}
+
+DeviceState* initialized_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        set_error(TVH_ERR_DEVICE, "no current HIP device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_devs[dev].device < 0) {
+        set_error(TVH_ERR_DEVICE, "device %d is not set up: call tvh_init(%d) first", dev, dev);
+        return nullptr;
+    }
+    return &g_devs[dev];
 }
 
 void* arena(DeviceState* ds, size_t bytes, hipStream_t st) {
@@ -164,6 +179,22 @@ int tvh_ntt_forward_batched_dev(uint64_t* d, size_t n, size_t cols, void* s) {
 int tvh_ntt_inverse_batched_dev(uint64_t* d, size_t n, size_t cols, void* s) {
     return ntt_batched_arena(d, n, cols, true, STREAM(s));
 }
+int tvh_ntt_batched_noalloc_dev(uint64_t* d, size_t n, size_t cols, int inverse, void* s) {
+    if (!d && cols) return set_error(TVH_ERR_INVALID_ARG, "null data");
+    if (cols == 0) return TVH_OK;
+    if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
+    if (n > kNoAllocScratchElems)
+        return set_error(TVH_ERR_UNSUPPORTED,
+                         "n = %zu exceeds the no-allocation NTT's scratch of %llu elements "
+                         "(use tvh_ntt_*_batched_dev)",
+                         n, (unsigned long long)kNoAllocScratchElems);
+    // set-up (tables, the scratch's address) must have happened: nothing here allocates
+    DeviceState* ds = initialized_device();
+    if (!ds) return TVH_ERR_DEVICE;
+    return launch_ntt_batched(d, n, cols, inverse != 0, ds->noalloc_scratch, kNoAllocScratchElems,
+                              STREAM(s));
+}
+
 int tvh_ntt_forward_dev(uint64_t* d, size_t n, void* s) {
     return tvh_ntt_forward_batched_dev(d, n, 1, s);
 }

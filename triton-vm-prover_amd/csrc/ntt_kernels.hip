@@ -374,6 +374,22 @@ int launch_lde_multi(int phase, const void* d_args, const u32* d_start, u32 nseg
     }
 }
 
+// Scratch of the allocation-free NTT entry point (tvh_ntt_batched_noalloc_dev):
+// one file-scope device array per GPU, held until the process ends.  Host
+// code never passes `scratch` itself to a kernel (in host code the name is a
+// host-side stand-in, not a device address): noalloc_scratch_setup resolves
+// this device's address once, at set-up, and launches take that pointer.
+static __device__ __align__(256) u64 scratch[kNoAllocScratchElems];
+
+int noalloc_scratch_setup(DeviceState* ds) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return set_error(TVH_ERR_DEVICE, "no-allocation scratch address: %s", hipGetErrorString(e));
+    ds->noalloc_scratch = (u64*)p;
+    return TVH_OK;
+}
+
 u64 ntt_scratch_elems(u64 n, u64 cols) {
     // columns go through the scratch in groups of at most 256 MB
     u64 group = cols;

@@ -39,8 +39,18 @@ struct DeviceState {
     // growable arena for entry points that need scratch (documented in tvh.h)
     void* arena = nullptr;
     size_t arena_bytes = 0;
+    // this device's address of the static scratch of the allocation-free NTT
+    u64* noalloc_scratch = nullptr;
 };
 
+// Elements of the static scratch behind tvh_ntt_batched_noalloc_dev (32 MiB).
+constexpr u64 kNoAllocScratchElems = 1ull << 22;
+// Resolves this device's address of that scratch (ntt_kernels.hip); called
+// once per device at set-up, with the device current.
+int noalloc_scratch_setup(DeviceState* ds);
+// The current device if tvh_init (or an earlier call) has set it up; nullptr
+// and an error otherwise.  Never allocates.
+DeviceState* initialized_device();
 
 // State of the current device, initialised on first use (thread-safe).
 // Returns nullptr and sets the error on failure.

@@ -89,6 +89,7 @@ _SIGS = {
     "tvh_last_error": [],
     "tvh_ntt_forward_batched_dev": [_vp, _sz, _sz, _vp],
     "tvh_ntt_inverse_batched_dev": [_vp, _sz, _sz, _vp],
+    "tvh_ntt_batched_noalloc_dev": [_vp, _sz, _sz, _i32, _vp],
     "tvh_ntt_forward_dev": [_vp, _sz, _vp],
     "tvh_ntt_inverse_dev": [_vp, _sz, _vp],
     "tvh_lde_batch_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _vp, _vp],
@@ -212,6 +213,10 @@ class TVH:
 
     def ntt_inverse_batched(self, d, n, cols, stream=None):
         self._call("tvh_ntt_inverse_batched_dev", _ptr(d), n, cols, self.stream_ptr(stream))
+
+    def ntt_batched_noalloc(self, d, n, cols, inverse, stream=None):
+        self._call("tvh_ntt_batched_noalloc_dev", _ptr(d), n, cols, int(inverse),
+                   self.stream_ptr(stream))
 
     # -- LDE --------------------------------------------------------------------
     def lde_batch(self, trace, cols, n, rand, k, toff, goff, m, out, stream=None):
