@@ -35,13 +35,31 @@ def hip():
             "hipEventElapsedTime": [C.POINTER(C.c_float), v, v],
             "hipMemGetInfo": [C.POINTER(sz), C.POINTER(sz)],
             "hipGetErrorString": [i],
+            "hipHostMalloc": [C.POINTER(v), sz, C.c_uint], "hipHostFree": [v],
+            "hipStreamWaitEvent": [v, v, C.c_uint],
         }
         for name, args in sigs.items():
             f = getattr(lib, name)
             f.argtypes = args
             f.restype = C.c_char_p if name == "hipGetErrorString" else C.c_int
-        _HIP = lib
+        _HIP = _Runtime(lib, sigs)
     return _HIP
+
+
+class _Runtime:
+    """The HIP runtime's declared entry points only.  ctypes passes the
+    arguments of an undeclared function as 32-bit ints, which truncates
+    pointers and sizes silently, so any other name raises instead."""
+
+    def __init__(self, lib, sigs):
+        self._lib = lib  # keeps the library loaded
+        self._fns = {name: getattr(lib, name) for name in sigs}
+
+    def __getattr__(self, name):
+        try:
+            return self.__dict__["_fns"][name]
+        except KeyError:
+            raise AttributeError(f"HIP runtime function {name} has no declared signature") from None
 
 
 H2D, D2H, D2D = 1, 2, 3
