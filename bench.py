@@ -124,8 +124,11 @@ def autotune(lib, shapes, tvh, force=0, skip=False):
             out[lg] = {"variant": tvh.lde_variant_name(force), "forced": True}
         elif not skip:
             chosen, ms = lib.lde_autotune(w, n, m)
-            out[lg] = {"variant": tvh.lde_variant_name(chosen),
-                       "ms": {tvh.lde_variant_name(k): round(v, 3) for k, v in ms.items()}}
+            ran = {k: v for k, v in ms.items() if v >= 0}  # -1: the variant cannot run the shape
+            slowest = max(ran, key=ran.get)
+            out[lg] = {"variant": tvh.lde_variant_name(chosen), "ms": round(ms[chosen], 3),
+                       "slowest": tvh.lde_variant_name(slowest), "slowest_ms": round(ran[slowest], 3),
+                       "variants_timed": len(ran)}
     return out
 
 
@@ -207,6 +210,7 @@ def run_main(args, tvh, lib, hip_rt):
     perms = perms_rows + (m - 1)
     achieved = lde_bytes / (lde_ms / 1000)
     tag = f"2p{args.log_n}x{w}"
+    counters = f"commit_{tag}_hbm_counters.md"
     out = {
         "metric": METRIC,
         "value": cells / (step_ms / 1000),
@@ -227,14 +231,20 @@ def run_main(args, tvh, lib, hip_rt):
         "root": root,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": profile_value(f"commit_{tag}_hbm_counters.md",
-                                              "traffic_bytes_per_launch"),
+                     "traffic": profile_value(counters, "traffic_bytes_per_launch"),
                      "kernel": lde_kernels(lib, n, m), "algorithmic_bytes": lde_bytes},
         "lde_forms": tuned,
     }
-    # Tip5: permutations/s; VALU issue fraction of the row-hash kernel from its
-    # measured VALU cycles per permutation (rocprof counters, profiles/)
-    cyc = profile_value(f"hash_rows_{tag}_counters.md", "valu_busy_cycles_per_permutation")
+    # VALU issue fractions against the chip's peak (1024 SIMDs x 2.4 GHz): the
+    # busy SIMD cycles per unit of work that the rocprof counters measured for
+    # these kernels (profiles/), over this run's event-timed phase durations
+    lde_cyc = profile_value(counters, "lde_valu_busy_cycles_per_cell")
+    if lde_cyc and not frugal:
+        out["roofline"]["valu"] = {
+            "busy_cycles_per_cell": lde_cyc,
+            "frac": lde_cyc * cells / (lde_ms / 1000) / (SIMDS * CLOCK_HZ),
+            "note": "the LDE kernels are VALU-issue bound: this is the roofline that binds them"}
+    cyc = profile_value(counters, "valu_busy_cycles_per_permutation")
     tip5 = {"permutations": perms, "permutations_per_s": perms / ((hash_ms + merkle_ms) / 1000),
             "row_hash_permutations_per_s": perms_rows / (hash_ms / 1000)}
     if cyc:
