@@ -33,7 +33,9 @@
  *    allocation and a wait on a private stream).  tvh_lde_prepare builds them
  *    ahead of time, so that later calls allocate nothing and never wait.
  * `_host` calls take host pointers, are synchronous and use device buffers of
- * their own (calls from several threads at once share nothing on the device).
+ * their own (calls from several threads at once share nothing on the device;
+ * their large uploads take turns on the page-locked staging ring of
+ * tvh_upload_dev).
  */
 #ifndef TVH_H
 #define TVH_H
@@ -375,6 +377,17 @@ int tvh_batch_plan_launch_graph(tvh_batch_plan* plan, void* stream);
 int tvh_batch_plan_destroy(tvh_batch_plan* plan);
 /* create + launch + wait for the stream + destroy. */
 int tvh_commit_batch_dev(const tvh_table_desc* tables, size_t count, void* stream);
+
+/* Copy count u64 from ordinary (pageable) host memory to the device on
+ * `stream`.  Large copies go through the library's page-locked staging ring
+ * (256 MiB per device, created on first use): worker threads fill its slots
+ * in parallel while earlier slots are in flight.  Returns once every piece is
+ * queued, so h_src may be reused at once and work queued later on `stream`
+ * sees the data.  Calls on one device take turns on the ring.  No reference
+ * counterpart: it replaces the prover's cudaMemcpy of host tables
+ * (main_gpu.cpp:282-322) when the host side keeps its tables in pageable
+ * memory. */
+int tvh_upload_dev(uint64_t* d_dst, const uint64_t* h_src, size_t count, void* stream);
 
 /* Synthetic uniform field elements for benchmarks and tests (no reference
  * counterpart): d[i] = SplitMix64(seed + (i+1) * golden gamma) reduced below p

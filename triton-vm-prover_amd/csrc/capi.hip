@@ -673,9 +673,8 @@ struct HostCall {
     }
     int up(u64* d, const u64* h, size_t elems) {
         if (!elems) return TVH_OK;
-        if (hipMemcpyAsync(d, h, elems * sizeof(u64), hipMemcpyHostToDevice, st) != hipSuccess)
-            return rc = set_error(TVH_ERR_DEVICE, "upload");
-        return TVH_OK;
+        int e = staged_h2d(d, h, elems * sizeof(u64), st);
+        return e ? (rc = e) : TVH_OK;
     }
     int down(u64* h, const u64* d, size_t elems) {
         if (!elems) return TVH_OK;

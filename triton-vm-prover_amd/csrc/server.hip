@@ -121,15 +121,13 @@ struct tvh_server {
         size_t tn = d.trace_len * d.num_cols, rn = d.rand_len * d.num_cols;
         int rc = TVH_OK;
         if (d.host_trace) {
-            if (tn && hipMemcpyAsync(tr, d.host_trace, 8 * tn, hipMemcpyHostToDevice, st) != hipSuccess)
-                rc = set_error(TVH_ERR_DEVICE, "trace upload");
+            rc = staged_h2d(tr, d.host_trace, 8 * tn, st);
         } else {
             rc = launch_fill_uniform(tr, tn, d.seed, st);
         }
         if (!rc && rn) {
             if (d.host_rand) {
-                if (hipMemcpyAsync(rd, d.host_rand, 8 * rn, hipMemcpyHostToDevice, st) != hipSuccess)
-                    rc = set_error(TVH_ERR_DEVICE, "randomizer upload");
+                rc = staged_h2d(rd, d.host_rand, 8 * rn, st);
             } else {
                 rc = launch_fill_uniform(rd, rn, d.seed ^ 0x5bd1e995ull, st);
             }

@@ -52,6 +52,11 @@ int noalloc_scratch_setup(DeviceState* ds);
 // and an error otherwise.  Never allocates.
 DeviceState* initialized_device();
 
+// Host-to-device copy of `bytes` from pageable host memory on stream st through
+// the device's page-locked staging ring (stager.hip); returns once every piece
+// is queued (the source may then be reused).  Calls on one device take turns.
+int staged_h2d(void* dst, const void* src, size_t bytes, hipStream_t st);
+
 // State of the current device, initialised on first use (thread-safe).
 // Returns nullptr and sets the error on failure.
 DeviceState* current_device();

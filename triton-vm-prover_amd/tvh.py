@@ -128,6 +128,7 @@ _SIGS = {
     "tvh_batch_plan_destroy": [_vp],
     "tvh_commit_batch_dev": [_vp, _sz, _vp],
     "tvh_fill_uniform_dev": [_vp, _sz, _u64, _vp],
+    "tvh_upload_dev": [_vp, _vp, _sz, _vp],
     "tvh_fill_table_dev": [_vp, _sz, _sz, _u64, _vp],
     "tvh_commit_frugal_workspace_elems": [_sz, _sz, _sz, _sz],
     "tvh_commit_frugal_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _sz, _vp, _vp, _i32, _vp],
@@ -357,6 +358,10 @@ class TVH:
 
     def fill_uniform(self, d, count, seed, stream=None):
         self._call("tvh_fill_uniform_dev", _ptr(d), count, seed, self.stream_ptr(stream))
+
+    def upload(self, d, host, count, stream=None):
+        """count u64 from host memory (numpy array or address) to device buffer d."""
+        self._call("tvh_upload_dev", _ptr(d), _ptr(host), count, self.stream_ptr(stream))
 
     def fill_table(self, d, rows, cols, seed, stream=None):
         self._call("tvh_fill_table_dev", _ptr(d), rows, cols, seed, self.stream_ptr(stream))
