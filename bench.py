@@ -740,7 +740,8 @@ def main() -> int:
     ap.add_argument("--jobs", type=int, default=2000, help="served jobs")
     ap.add_argument("--rate", type=float, default=40.0, help="mean arrivals/s of the served stream")
     ap.add_argument("--queries", type=int, default=80, help="leaf queries per served job")
-    ap.add_argument("--inflight", type=int, default=4, help="server stream slots")
+    ap.add_argument("--inflight", type=int, default=8,
+                    help="server stream slots (8: host uploads keep running while jobs compute)")
     ap.add_argument("--saturated-jobs", type=int, default=400, help="jobs of the capacity runs")
     ap.add_argument("--carried-jobs", type=int, default=400, help="jobs of the carried-load run")
     ap.add_argument("--no-autotune", action="store_true")
