@@ -272,6 +272,66 @@ def workload_name(args, n, w, E) -> str:
             f"row hashing, level-order Merkle tree")
 
 
+# ---------------------------------------------------------------- two-phase cosets
+def run_cosets(args, tvh, lib, hip_rt, reps=3):
+    """The prover's frugal main-table form (gpu_stark.cu:2356-2420): trace
+    coefficients once (phase 1), then per coset of the FRI domain 8N (98
+    randomizers per column, prover form): evaluate the coset, hash its rows,
+    scatter the digests; then the Merkle tree over all 8N leaves.  The
+    reference's source comments quote about 487 ms for its phase 1 and about
+    20 ms per coset evaluation (gpu_stark.cu:2368,2382), hardware unstated."""
+    P = 0xFFFFFFFF00000001
+    n, w, K, k = 1 << args.log_n, args.width, 8, 98
+    m = K * n
+    tr = hip_rt.DeviceBuffer(n * w)
+    lib.fill_table(tr, n, w, 1)
+    rand = hip_rt.DeviceBuffer(w * k)
+    lib.fill_uniform(rand, w * k, 98)
+    coeffs = hip_rt.DeviceBuffer(n * w)
+    work = hip_rt.DeviceBuffer(n * w)
+    dig = hip_rt.DeviceBuffer(5 * n)
+    tree = hip_rt.DeviceBuffer(5 * (2 * m - 1))
+    st = hip_rt.Stream()
+    wm = pow(7, (P - 1) // m, P)
+    gs = [7 * pow(wm, kk, P) % P for kk in range(K)]
+
+    def commit(ev=None):
+        if ev:
+            ev[0].record(st)
+        lib.trace_coefficients(tr, w, n, 1, coeffs, st)
+        if ev:
+            ev[1].record(st)
+        for kk in range(K):
+            lib.evaluate_coset(coeffs, w, n, rand, k, 1, gs[kk], work, st)
+            if ev and kk == 0:
+                ev[2].record(st)
+            lib.hash_bfield_rows(work, n, w, dig, st)
+            lib.scatter_digests(dig, tree, n, kk, K, st)
+        if ev:
+            ev[3].record(st)
+        lib.merkle_tree(tree, tree, m, 0, st)
+        if ev:
+            ev[4].record(st)
+
+    commit()
+    st.synchronize()
+    evs = [[hip_rt.Event() for _ in range(5)] for _ in range(reps)]
+    for i in range(reps):
+        commit(evs[i])
+    st.synchronize()
+    avg = lambda a, b: sum(e[a].elapsed_ms(e[b]) for e in evs) / reps
+    total = avg(0, 4)
+    root = [int(v) for v in tree.download(5, 5 * (2 * m - 2))]
+    return {"workload": f"frugal two-phase main-table commit 2^{args.log_n} x {w}, {k} randomizers "
+                        f"per column, FRI domain {K}N = 2^{(m).bit_length() - 1} rows in {K} cosets",
+            "phase1_trace_coefficients_ms": avg(0, 1),
+            "coset_evaluate_ms": avg(1, 2),
+            "coset_evaluate_hash_scatter_ms": (avg(1, 3)) / K,
+            "merkle_ms": avg(3, 4), "total_ms": total,
+            "cells_per_s": m * w / (total / 1000), "root": root,
+            "reference_comments_ms": {"phase1": 487, "per_coset_evaluation": 20}}
+
+
 # ---------------------------------------------------------------- batched tables
 class Pinned:
     """Page-locked host memory of u64 (hipHostMalloc)."""
@@ -731,8 +791,8 @@ def main() -> int:
     ap.add_argument("--mode", choices=["main", "aux", "frugal"], default="main",
                     help="headline table: main 2^20 x 379 (default); aux 2^20 x 88 XFE columns "
                          "(264 limbs); frugal 2^21 x 379 streamed in 32-column chunks")
-    ap.add_argument("--sections", default="main,batch,serve,cpu",
-                    help="comma list of main, batch, serve, cpu")
+    ap.add_argument("--sections", default="main,cosets,batch,serve,cpu",
+                    help="comma list of main, cosets, batch, serve, cpu")
     ap.add_argument("--tables", type=int, default=96, help="batched tables")
     ap.add_argument("--wave-gib", type=float, default=24.0,
                     help="HBM per wave of batched tables (smaller waves overlap host uploads better)")
@@ -765,6 +825,9 @@ def main() -> int:
         raise RuntimeError(f"tvh_init failed: {lib.status_string(rc)}: {lib.last_error()}")
     out = run_main(args, tvh, lib, hip_rt)
     print(f"bench: main {out['value']:.4g} cells/s", file=sys.stderr, flush=True)
+    if "cosets" in sections and args.mode == "main":
+        out["cosets"] = run_cosets(args, tvh, lib, hip_rt)
+        print(f"bench: cosets {out['cosets']['total_ms']:.1f} ms", file=sys.stderr, flush=True)
     hosts = None
     if ("batch" in sections or "serve" in sections) and args.mode == "main":
         # host traces for the host-ingest measurements: heights 2^12..2^21 at
