@@ -111,29 +111,36 @@ def cpu_baseline(width: int, expansion: int) -> dict:
 
 
 # ---------------------------------------------------------------- main table
+def width_class(w):
+    """The library's width classes for tuned LDE variants."""
+    return "<64" if w < 64 else "64..255" if w < 256 else ">=256"
+
+
 def autotune(lib, shapes, tvh, force=0, skip=False):
-    """The library's LDE autotuner once per (trace_len, target_len), timed on the
-    given width; returns {log2 trace_len: {"variant", "ms"}}."""
+    """The library's LDE autotuner once per (trace_len, target_len, width class),
+    timed on the first width of the class; returns {"log2 trace_len/width
+    class": {"variant", "ms", ...}}."""
     out = {}
     for n, w, m in sorted(set(shapes)):
         lg = n.bit_length() - 1
-        if lg in out:
+        key = f"{lg}/{width_class(w)}"
+        if key in out:
             continue
         if force:
             lib.lde_set_variant(lg, (m // n).bit_length() - 1, force)
-            out[lg] = {"variant": tvh.lde_variant_name(force), "forced": True}
+            out[key] = {"variant": tvh.lde_variant_name(force), "forced": True}
         elif not skip:
             chosen, ms = lib.lde_autotune(w, n, m)
             ran = {k: v for k, v in ms.items() if v >= 0}  # -1: the variant cannot run the shape
             slowest = max(ran, key=ran.get)
-            out[lg] = {"variant": tvh.lde_variant_name(chosen), "ms": round(ms[chosen], 3),
+            out[key] = {"variant": tvh.lde_variant_name(chosen), "ms": round(ms[chosen], 3),
                        "slowest": tvh.lde_variant_name(slowest), "slowest_ms": round(ran[slowest], 3),
                        "variants_timed": len(ran)}
     return out
 
 
-def lde_kernels(lib, n, m) -> str:
-    v = lib.lde_get_variant(n.bit_length() - 1, (m // n).bit_length() - 1)
+def lde_kernels(lib, n, m, w) -> str:
+    v = lib.lde_get_variant(n.bit_length() - 1, (m // n).bit_length() - 1, cols=w)
     tile = 14 - ((v >> 2) & 3)
     if v & 3 == 1:
         row = "k_lde_step2"
@@ -232,7 +239,7 @@ def run_main(args, tvh, lib, hip_rt):
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": profile_value(counters, "traffic_bytes_per_launch"),
-                     "kernel": lde_kernels(lib, n, m), "algorithmic_bytes": lde_bytes},
+                     "kernel": lde_kernels(lib, n, m, w), "algorithmic_bytes": lde_bytes},
         "lde_forms": tuned,
     }
     # VALU issue fractions against the chip's peak (1024 SIMDs x 2.4 GHz): the

@@ -176,7 +176,10 @@ int tvh_lde_release_constants(void);
  * gives only the inverse row kernel the 2^13 tile; TVH_LDE_ROW_QUARTER (with
  * TVH_LDE_SPLIT only, alone) gives both row kernels 2^12-element tiles (256
  * threads, four blocks per CU).  The variant is chosen per
- * (log2 trace_len, log2 expansion); TVH_LDE_AUTO restores the default
+ * (log2 trace_len, log2 expansion, width band), the width bands being
+ * tables of fewer than 64 columns, 64..255 and 256 or more;
+ * tvh_lde_set_variant sets every width band, tvh_lde_autotune the band of
+ * its num_cols.  TVH_LDE_AUTO restores the default
  * (TVH_LDE_SPLIT with column tile 2 and TVH_LDE_ROW_QUARTER for 2^13..2^16
  * rows, TVH_LDE_ROW_HALF for 2^17..2^18, column tile 1 and TVH_LDE_ROW_HALF
  * from 2^19; TVH_LDE_SPLIT | TVH_LDE_INV_HALF with tile 1 at 2^12;
@@ -193,11 +196,15 @@ enum {
     TVH_LDE_ROW_QUARTER = 64
 };
 int tvh_lde_set_variant(uint32_t log_n, uint32_t log_e, int variant);
+/* The variant used for tables of 256 or more columns. */
 int tvh_lde_get_variant(uint32_t log_n, uint32_t log_e, int* variant);
+/* The variant used for tables of num_cols columns. */
+int tvh_lde_get_variant_for_width(uint32_t log_n, uint32_t log_e, size_t num_cols, int* variant);
 /* Number of variants tvh_lde_autotune times (the length of its ms array). */
 int tvh_lde_variant_count(void);
 /* Times every variant on scratch buffers of this shape (num_cols capped so
- * the codewords stay under 4 GiB), keeps the fastest for the shape and returns
+ * the codewords stay under 4 GiB), keeps the fastest for the shape and the
+ * width band of num_cols, and returns
  * its code in *chosen; ms (may be NULL) receives the average milliseconds per
  * LDE of the variants 1, 2, 5, 6, 9, 10, 18, 22, 26, 34, 38, 42, 66, 70, 74 in
  * that order.  Synchronous; allocates

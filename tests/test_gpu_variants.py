@@ -141,9 +141,25 @@ def test_autotune_contract(T, variant):
     chosen, ms = T.lib.lde_autotune(8, 1 << 14, 1 << 16)
     variant(14, 4, chosen)  # restore afterwards
     assert chosen in VARIANTS
-    assert T.lib.lde_get_variant(14, 2) == chosen
+    assert T.lib.lde_get_variant(14, 2, cols=8) == chosen
     assert all(v > 0 for v in ms.values())
     assert ms[chosen] == min(ms.values())
+
+
+def test_autotune_keyed_by_width(T, variant):
+    """A choice is kept for the width class it was tuned on (< 64, 64..255,
+    >= 256 columns); setting a variant sets every class."""
+    variant(15, 4, 0)  # restored to the default afterwards
+    T.lib.lde_set_variant(15, 2, VARIANTS[0])
+    assert [T.lib.lde_get_variant(15, 2, cols=c) for c in (1, 100, 379)] == [VARIANTS[0]] * 3
+    chosen, _ = T.lib.lde_autotune(300, 1 << 15, 1 << 17)
+    assert T.lib.lde_get_variant(15, 2, cols=379) == chosen
+    assert T.lib.lde_get_variant(15, 2) == chosen  # the >= 256 class
+    assert T.lib.lde_get_variant(15, 2, cols=1) == VARIANTS[0]
+    assert T.lib.lde_get_variant(15, 2, cols=100) == VARIANTS[0]
+    chosen8, _ = T.lib.lde_autotune(8, 1 << 15, 1 << 17)
+    assert T.lib.lde_get_variant(15, 2, cols=8) == chosen8
+    assert T.lib.lde_get_variant(15, 2, cols=379) == chosen
 
 
 def test_autotune_skips_unsupported_variants(T, variant):

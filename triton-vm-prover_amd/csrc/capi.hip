@@ -258,9 +258,13 @@ static constexpr int kNumLdeVariants = sizeof(kLdeVariants) / sizeof(kLdeVariant
 int tvh_lde_variant_count(void) { return kNumLdeVariants; }
 
 int tvh_lde_get_variant(uint32_t log_n, uint32_t log_e, int* variant) {
+    return tvh_lde_get_variant_for_width(log_n, log_e, 256, variant);
+}
+
+int tvh_lde_get_variant_for_width(uint32_t log_n, uint32_t log_e, size_t num_cols, int* variant) {
     if (!variant) return set_error(TVH_ERR_INVALID_ARG, "null variant pointer");
     if (log_n > kMaxLogN || log_e > 4) return set_error(TVH_ERR_INVALID_ARG, "shape");
-    *variant = lde_variant(log_n, log_e);
+    *variant = lde_variant(log_n, log_e, num_cols);
     return TVH_OK;
 }
 
@@ -288,11 +292,15 @@ int tvh_lde_autotune(size_t cols, size_t n, size_t m, int* chosen, float* ms) {
     if (!rc) rc = launch_fill_uniform(trace, n * tc, 77, st);
     float best = 0;
     int best_v = LDE_WIDE;
-    int prev = lde_variant(logN, logE);
+    // the variants are timed on tc columns (their width class); the choice is
+    // kept for the caller's width class
+    const int wc_timed = width_class(tc), wc = width_class(cols);
+    int prev_timed = lde_variant(logN, logE, tc);
+    int prev = lde_variant(logN, logE, cols);
     if (!rc) rc = lde_prepare_consts(n, m, 1, 7);
     bool any = false;
     for (int i = 0; i < kNumLdeVariants && !rc; i++) {
-        set_lde_variant(logN, logE, kLdeVariants[i]);
+        set_lde_variant(logN, logE, kLdeVariants[i], wc_timed);
         RandSpec none;
         rc = launch_lde(trace, tc, n, none, 1, 7, m, false, out, st);  // warm-up
         if (rc == TVH_ERR_UNSUPPORTED) {  // this variant cannot run the shape: skip it
@@ -315,7 +323,8 @@ int tvh_lde_autotune(size_t cols, size_t n, size_t m, int* chosen, float* ms) {
         }
     }
     if (!rc && !any) rc = set_error(TVH_ERR_UNSUPPORTED, "no LDE variant runs this shape");
-    set_lde_variant(logN, logE, rc ? prev : best_v);
+    set_lde_variant(logN, logE, prev_timed, wc_timed);
+    set_lde_variant(logN, logE, rc ? prev : best_v, wc);
     if (chosen) *chosen = rc ? prev : best_v;
     if (st) (void)hipStreamSynchronize(st);
     if (e0) (void)hipEventDestroy(e0);

@@ -60,11 +60,11 @@ __global__ void k_xfe_relayout(const u64* __restrict__ src, u64* __restrict__ ds
 
 // ---------------------------------------------------------------- host side
 
-// Tuned K2 form per (log2 N, log2 E); 0 = not set.
-static std::atomic<int> g_variant[kMaxLogN + 1][5];
+// Tuned variant per (log2 N, log2 E, width class); 0 = not set.
+static std::atomic<int> g_variant[kMaxLogN + 1][5][kWidthClasses];
 
-int lde_variant(u32 logN, u32 logE) {
-    int v = (logN <= kMaxLogN && logE <= 4) ? g_variant[logN][logE].load() : 0;
+int lde_variant(u32 logN, u32 logE, u64 cols) {
+    int v = (logN <= kMaxLogN && logE <= 4) ? g_variant[logN][logE][width_class(cols)].load() : 0;
     if (v) return v;
     // default (measured on MI355X at widths 156..379, 2^12..2^21 rows): the split
     // form with 2^12-element row tiles for 2^13..2^16 rows, 2^13-element row
@@ -84,10 +84,11 @@ bool lde_variant_valid(int v) {
                              half + ihalf + quarter <= (form == LDE_SPLIT));
 }
 
-int set_lde_variant(u32 logN, u32 logE, int v) {
-    if (logN > kMaxLogN || logE > 4 || !lde_variant_valid(v))
+int set_lde_variant(u32 logN, u32 logE, int v, int wc) {
+    if (logN > kMaxLogN || logE > 4 || !lde_variant_valid(v) || wc >= kWidthClasses)
         return set_error(TVH_ERR_INVALID_ARG, "lde variant (%u, %u, %d)", logN, logE, v);
-    g_variant[logN][logE].store(v);
+    for (int c = 0; c < kWidthClasses; c++)
+        if (wc < 0 || c == wc) g_variant[logN][logE][c].store(v);
     return TVH_OK;
 }
 
@@ -228,7 +229,7 @@ int lde_release_consts() {
 int prepare_lde(const u64* trace, u64 cols, u64 n, const RandSpec& rs, u64 toff, u64 goff,
                 u64 m, bool truncate, u64* out, LdePlan* P, bool batch) {
     P->k1 = P->k3 = P->split = false;
-    P->k2_variant = lde_variant(log2u(n), log2u(m / (n ? n : 1)));
+    P->k2_variant = lde_variant(log2u(n), log2u(m / (n ? n : 1)), cols);
     // multi-table launches run the column and row kernels with one tile each
     // (kBatchColLt, kBatchRowLt) for every table of the launch
     P->col_lt = batch ? kBatchColLt : LOG_TILE - ((P->k2_variant >> 2) & 3);

@@ -151,9 +151,15 @@ enum LdeVariant {
     LDE_INV_HALF = 32,
     LDE_ROW_QUARTER = 64
 };
-int lde_variant(u32 logN, u32 logE);            // the tuned choice (never LDE_AUTO)
+// Tuned choices are kept per (log2 N, log2 E, width class): tables narrower
+// than 64 columns, 64..255, and 256 or more (a narrow table launches fewer
+// blocks per kernel, which can favour a smaller tile).
+constexpr int kWidthClasses = 3;
+inline int width_class(u64 cols) { return cols < 64 ? 0 : cols < 256 ? 1 : 2; }
+int lde_variant(u32 logN, u32 logE, u64 cols);  // the tuned choice (never LDE_AUTO)
 bool lde_variant_valid(int v);
-int set_lde_variant(u32 logN, u32 logE, int v);  // v = LDE_AUTO restores the default
+// v = LDE_AUTO restores the default; wc < 0 sets every width class
+int set_lde_variant(u32 logN, u32 logE, int v, int wc = -1);
 
 struct LdePlan {
     bool k1, k3;

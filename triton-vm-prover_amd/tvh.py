@@ -134,6 +134,7 @@ _SIGS = {
     "tvh_commit_frugal_dev": [_vp, _sz, _sz, _vp, _sz, _u64, _u64, _sz, _sz, _vp, _vp, _i32, _vp],
     "tvh_lde_set_variant": [C.c_uint32, C.c_uint32, _i32],
     "tvh_lde_get_variant": [C.c_uint32, C.c_uint32, _vp],
+    "tvh_lde_get_variant_for_width": [C.c_uint32, C.c_uint32, _sz, _vp],
     "tvh_lde_autotune": [_sz, _sz, _sz, _vp, _vp],
     "tvh_lde_variant_count": [],
     "tvh_server_create": [_vp, _vp],
@@ -378,9 +379,12 @@ class TVH:
     def lde_set_variant(self, log_n, log_e, variant):
         self._call("tvh_lde_set_variant", log_n, log_e, variant)
 
-    def lde_get_variant(self, log_n, log_e):
+    def lde_get_variant(self, log_n, log_e, cols=None):
         v = C.c_int()
-        self._call("tvh_lde_get_variant", log_n, log_e, C.byref(v))
+        if cols is None:
+            self._call("tvh_lde_get_variant", log_n, log_e, C.byref(v))
+        else:
+            self._call("tvh_lde_get_variant_for_width", log_n, log_e, cols, C.byref(v))
         return v.value
 
     def lde_autotune(self, cols, n, m):
