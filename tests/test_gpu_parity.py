@@ -334,8 +334,9 @@ def test_commit_frugal_matches_plain(T, oracle, w, chunk, k):
 
 def test_commit_frugal_full_size(T, oracle):
     """Frugal-mode shape 2^21 x 379 -> 2^23 in 32-column chunks: the whole tree (every digest
-    and the root) equals the plain commit's, whose first codeword column is checked against the
-    oracle."""
+    and the root) equals the plain commit's, whose codewords are checked against the oracle:
+    the first column in full and every column at sampled rows (the oracle's interpolant
+    evaluated at 7 w_M^j), with the sampled rows' digests against the oracle's hash."""
     n, w, E, chunk = 1 << 21, 379, 4, 32
     m = E * n
     dt = T.empty(n * w)
@@ -344,7 +345,15 @@ def test_commit_frugal_full_size(T, oracle):
     tree = T.empty(5 * (2 * m - 1))
     T.lib.commit(dt, w, n, None, 0, 1, 7, m, cw, tree, 0)
     T.sync()
-    assert np.array_equal(cw.download(m, 0), oracle.lde_column(dt.download(n, 0), m))
+    trace = dt.download()
+    assert np.array_equal(cw.download(m, 0), oracle.lde_column(trace[:n], m))
+    rows = [0, 3, m // 2 + 7, m - 1, 5234567]
+    want = oracle.lde_sample(trace, n, w, m, rows)
+    del trace
+    for q, r in enumerate(rows):
+        vals = cw.gather_strided(r, m, w)
+        assert np.array_equal(vals, want[:, q]), r
+        assert np.array_equal(tree.download(5, 5 * r), oracle.hash_varlen(vals)), r
     del cw
     work = T.empty(T.lib.frugal_workspace_elems(w, n, m, chunk))
     tree2 = T.empty(5 * (2 * m - 1))
