@@ -77,7 +77,7 @@ def cpu_model() -> str:
 def cpu_baseline(width: int, expansion: int) -> dict:
     """The reference's own CPU commit (oracle/_ref, compiled from the reference
     sources): the parallel form (OpenMP over columns and rows, Merkle levels) on
-    a 2^16-row sample with every host thread, and the single-thread form on a
+    a 2^17-row sample with every host thread, and the single-thread form on a
     2^13-row sample.  Fails when oracle/_ref is not built."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, Reference
@@ -88,7 +88,7 @@ def cpu_baseline(width: int, expansion: int) -> dict:
     orc, ref = Oracle(), Reference()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     out = {}
-    for lg, th in ((16, threads), (13, 1)):
+    for lg, th in ((17, threads), (13, 1)):
         n, m = 1 << lg, expansion << lg
         trace = orc.table_rowmajor_draw(1, n, width)
         ref.set_threads(th)
