@@ -376,6 +376,21 @@ int tvh_lde_xfe_batch_dev(const uint64_t* xt, size_t xcols, size_t n, const uint
 }
 
 // ---------------------------------------------------------------- two-phase coset evaluation
+// The power tables of the two-phase scaling pass, in the per-device arena
+// after the batched NTT's scratch (the arena is sized for both, so the NTT
+// that follows on the stream finds it large enough and leaves the tables be).
+static u64* coset_tables(size_t n, size_t cols, hipStream_t st) {
+    DeviceState* ds = current_device();
+    if (!ds) return nullptr;
+    if (n > (1ull << kMaxLogN)) {
+        set_error(TVH_ERR_UNSUPPORTED, "trace_len > 2^%u", kMaxLogN);
+        return nullptr;
+    }
+    u64 scratch = ntt_scratch_elems(n, cols);
+    u64* base = (u64*)arena(ds, (scratch + scale_table_elems(n)) * sizeof(u64), st);
+    return base ? base + scratch : nullptr;
+}
+
 int tvh_trace_coefficients_dev(const uint64_t* trace, size_t cols, size_t n, uint64_t toff,
                                uint64_t* coeffs, void* s) {
     if (cols && (!trace || !coeffs)) return set_error(TVH_ERR_INVALID_ARG, "null buffer");
@@ -383,12 +398,14 @@ int tvh_trace_coefficients_dev(const uint64_t* trace, size_t cols, size_t n, uin
     if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "trace_len must be a power of 2");
     if (toff == 0 || toff >= TVH_P) return set_error(TVH_ERR_INVALID_ARG, "trace offset must be canonical and nonzero");
     hipStream_t st = STREAM(s);
+    u64* tables = nullptr;
+    if (toff != 1 && !(tables = coset_tables(n, cols, st))) return TVH_ERR_OOM;
     if (coeffs != trace &&
         hipMemcpyAsync(coeffs, trace, 8 * n * cols, hipMemcpyDeviceToDevice, st) != hipSuccess)
         return set_error(TVH_ERR_DEVICE, "coefficient copy");
     int rc = ntt_batched_arena(coeffs, n, cols, true, st);
     if (rc || toff == 1) return rc;
-    return launch_scale_powers(coeffs, coeffs, n, cols, gl::inv(toff), nullptr, 0, 0, st);
+    return launch_scale_table(coeffs, coeffs, n, cols, gl::inv(toff), nullptr, 0, 0, tables, st);
 }
 
 int tvh_evaluate_coset_dev(const uint64_t* coeffs, size_t cols, size_t n, const uint64_t* rand,
@@ -402,8 +419,11 @@ int tvh_evaluate_coset_dev(const uint64_t* coeffs, size_t cols, size_t n, const 
     if (k > n) return set_error(TVH_ERR_UNSUPPORTED, "randomizer_len > trace_len");
     if (toff >= TVH_P || coset_offset >= TVH_P) return set_error(TVH_ERR_INVALID_ARG, "offsets must be canonical");
     hipStream_t st = STREAM(s);
+    u64* tables = coset_tables(n, cols, st);
+    if (!tables) return TVH_ERR_OOM;
     u64 delta = gl::sub(gl::pow(coset_offset, n), gl::pow(toff, n));
-    int rc = launch_scale_powers(coeffs, out, n, cols, coset_offset, k ? rand : nullptr, k, delta, st);
+    int rc = launch_scale_table(coeffs, out, n, cols, coset_offset, k ? rand : nullptr, k, delta,
+                                tables, st);
     if (rc) return rc;
     return ntt_batched_arena(out, n, cols, false, st);
 }

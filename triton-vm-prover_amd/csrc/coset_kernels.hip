@@ -8,39 +8,42 @@
 // On the coset g<w_n> the factor X^n is the constant g^n, so
 //   P(g w_n^j) = NTT_n(b)_j,  b_i = (c_i + [i < k] r_i (g^n - o^n)) g^i
 // and phase 2 is one scaling pass plus one forward NTT (the reference adds a
-// second NTT for the randomizer tail instead).
+// second NTT for the randomizer tail instead).  The scaling pass reads g^i
+// from two small per-call tables (g^i = g^(i mod 4096) * (g^4096)^(i / 4096)).
 #include "tvh_internal.hpp"
 
 namespace tvh {
 
 namespace {
 
-constexpr int kScaleTpb = 256;
-constexpr int kScaleRun = 8;  // consecutive elements per thread (one power, then a ratio)
+// t[i] = base^i in Montgomery form (base^i * 2^64 mod p), i < count: one
+// square-and-multiply per entry (a table of at most 4096 entries per call).
+__global__ void k_pow_table(u64* __restrict__ t, u64 base, u32 count) {
+    u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) t[i] = gl::canon(gl::to_mont(gl::pow(base, i)));
+}
 
-// y[c*n + i] = (x[c*n + i] + [i < k] rand[c*k + i] * delta) * base^i, for
-// i in a run of kScaleRun; base^i0 by square-and-multiply, then one product
-// per element.  x may equal y.
-__global__ void __launch_bounds__(kScaleTpb) k_scale_powers(const u64* x, u64* y, u64 n, u64 cols,
-                                                            u64 base, const u64* __restrict__ rand,
-                                                            u64 k, u64 delta) {
-    u64 run = (u64)blockIdx.x * kScaleTpb + threadIdx.x;
-    u64 runs_per_col = (n + kScaleRun - 1) / kScaleRun;
-    if (run >= runs_per_col * cols) return;
-    u64 c = run / runs_per_col;
-    u64 i0 = (run - c * runs_per_col) * kScaleRun;
-    u64 p = gl::pow(base, i0);
-    const u64* xc = x + c * n;
-    u64* yc = y + c * n;
+constexpr int kTabTpb = 256;
+constexpr int kTabPer = 4;  // elements per thread, kTabTpb apart (coalesced)
+
+// y[c*n + i] = (x[c*n + i] + [i < k] rand[c*k + i] * delta) * base^i with
+// base^i = tlo[i mod 4096] * thi[i / 4096] (Montgomery-form tables): two
+// cached loads and two products per element.  x may equal y.
+__global__ void __launch_bounds__(kTabTpb) k_scale_table(const u64* x, u64* y, u64 n, u32 log_n,
+                                                         u64 total, const u64* __restrict__ tlo,
+                                                         const u64* __restrict__ thi,
+                                                         const u64* __restrict__ rand, u64 k,
+                                                         u64 delta) {
+    const u64 e0 = (u64)blockIdx.x * (kTabTpb * kTabPer) + threadIdx.x;
 #pragma unroll
-    for (int t = 0; t < kScaleRun; t++) {
-        u64 i = i0 + t;
-        if (i < n) {
-            u64 v = xc[i];
-            if (i < k) v = gl::add(v, gl::mul(rand[c * k + i], delta));
-            yc[i] = gl::mul(v, p);
-            p = gl::mul(p, base);
-        }
+    for (int r = 0; r < kTabPer; r++) {
+        u64 e = e0 + (u64)r * kTabTpb;
+        if (e >= total) break;
+        u64 i = e & (n - 1);
+        u64 v = x[e];
+        if (i < k) v = gl::add(v, gl::mul(rand[(e >> log_n) * k + i], delta));
+        u64 f = gl::mont_mul(tlo[i & 4095], thi[i >> 12]);  // base^i in Montgomery form
+        y[e] = gl::mont_mul(v, f);
     }
 }
 
@@ -141,12 +144,24 @@ int launch_fill_table(u64* d, u64 rows, u64 cols, u64 seed, u64* rej, hipStream_
 
 u64 fill_table_scratch_elems() { return 1 + kMaxRejects; }
 
-int launch_scale_powers(const u64* x, u64* y, u64 n, u64 cols, u64 base, const u64* rand, u64 k,
-                        u64 delta, hipStream_t st) {
+u64 scale_table_elems(u64 n) { return 4096 + (n + 4095) / 4096; }
+
+int launch_scale_table(const u64* x, u64* y, u64 n, u64 cols, u64 base, const u64* rand, u64 k,
+                       u64 delta, u64* tables, hipStream_t st) {
     if (!n || !cols) return TVH_OK;
-    u64 runs = (n + kScaleRun - 1) / kScaleRun * cols;
-    hipLaunchKernelGGL(k_scale_powers, dim3((u32)((runs + kScaleTpb - 1) / kScaleTpb)),
-                       dim3(kScaleTpb), 0, st, x, y, n, cols, base, rand, k, delta);
+    if (n > (1ull << 24)) return set_error(TVH_ERR_UNSUPPORTED, "scale table for n > 2^24");
+    const u32 nlo = (u32)(n < 4096 ? n : 4096), nhi = (u32)((n + 4095) / 4096);
+    u64* tlo = tables;
+    u64* thi = tables + 4096;
+    hipLaunchKernelGGL(k_pow_table, dim3((nlo + 255) / 256), dim3(256), 0, st, tlo, base, nlo);
+    hipLaunchKernelGGL(k_pow_table, dim3((nhi + 255) / 256), dim3(256), 0, st, thi,
+                       gl::pow(base, 4096), nhi);
+    const u64 total = n * cols;
+    const u64 blocks = (total + kTabTpb * kTabPer - 1) / (kTabTpb * kTabPer);
+    if (blocks > 0x7FFFFFFFull) return set_error(TVH_ERR_UNSUPPORTED, "scale of %llu elements",
+                                                 (unsigned long long)total);
+    hipLaunchKernelGGL(k_scale_table, dim3((u32)blocks), dim3(kTabTpb), 0, st, x, y, n, log2u(n), total,
+                       tlo, thi, rand, k, delta);
     return check_launch();
 }
 

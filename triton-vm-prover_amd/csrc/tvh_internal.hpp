@@ -204,9 +204,11 @@ int launch_xfe_interleave(const u64* src, u64* dst, u64 cols, u64 len, bool to_l
                           hipStream_t st);
 
 // ---- coset evaluation, digest scatter, transposes (coset_kernels.hip) ----
-// y[c*n + i] = (x[c*n + i] + [i < k] rand[c*k + i] * delta) * base^i (x may be y)
-int launch_scale_powers(const u64* x, u64* y, u64 n, u64 cols, u64 base, const u64* rand, u64 k,
-                        u64 delta, hipStream_t st);
+// y[c*n + i] = (x[c*n + i] + [i < k] rand[c*k + i] * delta) * base^i (x may be y),
+// n <= 2^24; tables: scale_table_elems(n) u64 of device scratch for the powers
+u64 scale_table_elems(u64 n);
+int launch_scale_table(const u64* x, u64* y, u64 n, u64 cols, u64 base, const u64* rand, u64 k,
+                       u64 delta, u64* tables, hipStream_t st);
 // leaves[(j * num_cosets + coset) * 5 + e] = dig[j * 5 + e], j < len
 int launch_scatter_digests(const u64* dig, u64* leaves, u64 len, u64 coset, u64 num_cosets,
                            hipStream_t st);
