@@ -268,31 +268,37 @@ __device__ __forceinline__ void fused_pass(const LdsIO& io, int logL, int NT, in
 // first/last role is a compile-time constant, which keeps register pressure
 // at one pass's worth.  The twiddles of pass P + 1 are loaded before the
 // barrier that ends pass P, so their latency overlaps the barrier wait.
-template <int P, int NP, int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, class Ld,
-          class St>
+// The last pass maps threads by TFL, the others by TF: a row transform whose
+// output is stored transposed (adjacent rows adjacent in memory) loads its
+// rows element-fastest and stores them transform-fastest, both coalesced.
+template <int P, int NP, int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, bool TFL,
+          class Ld, class St>
 __device__ __forceinline__ void fused_chain(const LdsIO& io, int NT, int logNT,
                                             const u64* __restrict__ r4096, const Ld& ld,
                                             const St& st,
                                             const PassTw<(P < Q ? 4 : REM), NTPB, NTILE>& tw) {
     constexpr int logL = 4 * Q + REM;
-    fused_pass<(P < Q ? 4 : REM), INV, NTPB, NTILE, TF, (P > 0), (P + 1 < NP)>(
+    constexpr bool tf = P + 1 == NP ? TFL : TF;
+    fused_pass<(P < Q ? 4 : REM), INV, NTPB, NTILE, tf, (P > 0), (P + 1 < NP)>(
         io, logL, NT, logNT, 1 << (4 * P), tw, ld, st);
     if constexpr (P + 1 < NP) {
+        constexpr bool tf_next = P + 2 == NP ? TFL : TF;
         PassTw<(P + 1 < Q ? 4 : REM), NTPB, NTILE> next;
-        prefetch_tw<(P + 1 < Q ? 4 : REM), NTPB, NTILE, TF>(next, logL, NT, logNT, 1 << (4 * (P + 1)),
-                                                           r4096);
+        prefetch_tw<(P + 1 < Q ? 4 : REM), NTPB, NTILE, tf_next>(next, logL, NT, logNT,
+                                                                1 << (4 * (P + 1)), r4096);
         lds_barrier();
-        fused_chain<P + 1, NP, Q, REM, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st, next);
+        fused_chain<P + 1, NP, Q, REM, INV, NTPB, NTILE, TF, TFL>(io, NT, logNT, r4096, ld, st,
+                                                                  next);
     }
 }
 
-template <int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, class Ld, class St>
+template <int Q, int REM, bool INV, int NTPB, int NTILE, bool TF, bool TFL, class Ld, class St>
 __device__ __forceinline__ void fused_passes(const LdsIO& io, int NT, int logNT,
                                              const u64* __restrict__ r4096, const Ld& ld,
                                              const St& st) {
     constexpr int NP = Q + ((REM != 0 || Q == 0) ? 1 : 0);
     PassTw<(0 < Q ? 4 : REM), NTPB, NTILE> none;  // the first pass has no twiddles
-    fused_chain<0, NP, Q, REM, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st, none);
+    fused_chain<0, NP, Q, REM, INV, NTPB, NTILE, TF, TFL>(io, NT, logNT, r4096, ld, st, none);
 }
 
 // LDS row pitch of a transform for the fused DFT: L + L/16 + pad, with pad
@@ -305,14 +311,15 @@ __host__ __device__ inline int fused_pitch(int L, int NT, bool tf) {
 }
 
 // NT (power of two) transforms of length 2^logL (<= 4096), NT * L <= NTILE.
-// The block's LDS (buf, NT * fused_pitch(L, NT, TF) u64) must not be in use by
-// other threads on entry.
-template <bool INV, int NTPB, int NTILE, bool TF, class Ld, class St>
+// The block's LDS (buf, NT * fused_pitch(L, NT, TF || TFL) u64) must not be in
+// use by other threads on entry.  TFL: thread mapping of the last pass.
+template <bool INV, int NTPB, int NTILE, bool TF, bool TFL = TF, class Ld, class St>
 __device__ __forceinline__ void fused_dft(u64* buf, int logL, int NT, int logNT,
                                           const u64* __restrict__ r4096, const Ld& ld,
                                           const St& st) {
-    const LdsIO io{buf, fused_pitch(1 << logL, NT, TF)};
-#define TVH_FUSED(q, r) fused_passes<q, r, INV, NTPB, NTILE, TF>(io, NT, logNT, r4096, ld, st)
+    const LdsIO io{buf, fused_pitch(1 << logL, NT, TF || TFL)};
+#define TVH_FUSED(q, r) \
+    fused_passes<q, r, INV, NTPB, NTILE, TF, TFL>(io, NT, logNT, r4096, ld, st)
     switch (logL) {
         case 0: TVH_FUSED(0, 0); break;
         case 1: TVH_FUSED(0, 1); break;
@@ -768,7 +775,7 @@ int launch_k_step1(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, con
 int launch_k_step1_multi(const Step1Args* args, const u32* start, u32 nseg, u32 blocks, size_t lds,
                          hipStream_t st, const Roots& rt);
 int launch_k_ntt_step2(bool inv, u32 blocks, size_t lds, hipStream_t st, const u64* src, u64* dst,
-                       u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv, const Roots& rt);
+                       u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv_m, const Roots& rt);
 int set_attributes_step1(int lds_bytes);
 // lde_rows.hip
 int launch_k_lde_rowinv(u32 lt, u32 blocks, size_t lds, hipStream_t st, const LdeArgs& A,

@@ -419,7 +419,7 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, u64* scratch, u
         return set_error(TVH_ERR_INVALID_ARG, "NTT scratch of %llu elements < n = %llu",
                          (unsigned long long)scratch_elems, (unsigned long long)n);
     if (group > cols) group = cols;
-    u64 ninv = gl::inv(n % gl::P);
+    u64 ninv_m = gl::mul(gl::inv(n % gl::P), gl::EPS);  // N^-1 in Montgomery form
     int rc;
     for (u64 c0 = 0; c0 < cols; c0 += group) {
         u64 g = cols - c0 < group ? cols - c0 : group;
@@ -439,8 +439,10 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, u64* scratch, u
             u64 nrows = g << a;
             if ((1ull << logNT) > nrows) logNT = log2u(nrows);
             u64 blocks = (nrows + (1ull << logNT) - 1) >> logNT;
-            if ((rc = launch_k_ntt_step2(inverse, (u32)blocks, lds_bytes(b, logNT), st, scratch, d,
-                                         logN, a, nrows, logNT, ninv, ds->roots)))
+            const size_t s2_lds =
+                (size_t)fused_pitch(1 << b, 1 << logNT, true) * (1u << logNT) * sizeof(u64);
+            if ((rc = launch_k_ntt_step2(inverse, (u32)blocks, s2_lds, st, scratch, d, logN, a,
+                                         nrows, logNT, ninv_m, ds->roots)))
                 return rc;
         }
     }

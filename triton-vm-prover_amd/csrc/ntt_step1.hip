@@ -18,33 +18,61 @@ __global__ void __launch_bounds__(1 << (LT - 4), 4) k_step1_multi(const Step1Arg
     step1_body<INV, LT>(A, blockIdx.x - start[seg], rt);
 }
 
+// Plain NTT row step.  Rows s = (c, k1) of N2 = 2^b elements at src[s*N2 + n2]
+// (the column step's output); DFT over n2 -> k2, times N^-1 when INV; element
+// k2 of row (c, k1) to dst[c*N + k1 + N1*k2].  Fused endpoints: the first pass
+// reads the rows from HBM element-fastest, the last pass writes
+// transform-fastest, so adjacent lanes write adjacent k1 (runs as long as the
+// block's row count).  Offsets stay below N <= 2^24 plus one tile.
+struct PlainRowLoad {
+    const u64* src;
+    u64 s0, nrows;
+    u32 b;
+    template <int R>
+    __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
+        if (s0 + t < nrows) {
+            const u64* p = src + (s0 << b);
+            const u32 o = (u32)t << b;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = p[o + (u32)(j + r * per)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = 0;
+        }
+    }
+};
+template <bool INV>
+struct PlainRowStore {
+    u64* dst;
+    u64 n, s0, nrows, ninv_m;  // ninv_m: N^-1 in Montgomery form
+    u32 a;
+    template <int R>
+    __device__ __forceinline__ void store(int t, int j, int Ns, const u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s >= nrows) return;
+        const u64 mask = (1ull << a) - 1;
+        u64* p = dst + (s0 >> a) * n + (s0 & mask);
+        const u32 o = (u32)(((s >> a) - (s0 >> a)) * n + (s & mask) - (s0 & mask));
+        int k = j & (Ns - 1), base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            // (canonical in, canonical out: the product is below p^2)
+            u64 x = INV ? gl::mont_mul(v[r], ninv_m) : v[r];
+            p[o + ((u32)(base + r * Ns) << a)] = x;
+        }
+    }
+};
+
 template <bool INV>
 __global__ void __launch_bounds__(TPB) k_ntt_step2(const u64* __restrict__ src,
                                                    u64* __restrict__ dst, u32 logN, u32 a,
-                                                   u64 nrows, u32 logNT, u64 ninv, Roots rt) {
+                                                   u64 nrows, u32 logNT, u64 ninv_m, Roots rt) {
     extern __shared__ u64 lds[];
     const u32 b = logN - a;
-    const int L = 1 << b;
-    const int Lp = padded_len(L);
-    const int NT = 1 << logNT;
     const u64 s0 = (u64)blockIdx.x << logNT;
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
-        int t = e >> b, i = e & (L - 1);
-        u64 s = s0 + t;
-        if (s < nrows) lds[pidx(t, i, Lp)] = src[(s << b) + i];
-    }
-    __syncthreads();
-    lds_dft<INV>(lds, b, NT, rt.r4096[INV]);
-    for (int e = threadIdx.x; e < NT * L; e += TPB) {
-        int t = e & (NT - 1), k2 = e >> logNT;
-        u64 s = s0 + t;
-        if (s < nrows) {
-            u64 c = s >> a, k1 = s & ((1ull << a) - 1);
-            u64 v = lds[pidx(t, k2, Lp)];
-            if (INV) v = gl::mul(v, ninv);
-            dst[(c << logN) + k1 + ((u64)k2 << a)] = v;
-        }
-    }
+    const PlainRowLoad ld{src, s0, nrows, b};
+    const PlainRowStore<INV> st{dst, 1ull << logN, s0, nrows, ninv_m, a};
+    fused_dft<INV, TPB, TILE, false, true>(lds, b, 1 << logNT, logNT, rt.r4096[INV], ld, st);
 }
 
 int launch_k_step1(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, const Step1Args& A,
@@ -71,13 +99,13 @@ int launch_k_step1_multi(const Step1Args* args, const u32* start, u32 nseg, u32 
 }
 
 int launch_k_ntt_step2(bool inv, u32 blocks, size_t lds, hipStream_t st, const u64* src, u64* dst,
-                       u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv, const Roots& rt) {
+                       u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv_m, const Roots& rt) {
     if (inv)
         hipLaunchKernelGGL(k_ntt_step2<true>, dim3(blocks), dim3(TPB), lds, st, src, dst, logN, a,
-                           nrows, logNT, ninv, rt);
+                           nrows, logNT, ninv_m, rt);
     else
         hipLaunchKernelGGL(k_ntt_step2<false>, dim3(blocks), dim3(TPB), lds, st, src, dst, logN, a,
-                           nrows, logNT, ninv, rt);
+                           nrows, logNT, ninv_m, rt);
     return check_launch();
 }
 
