@@ -392,9 +392,13 @@ int noalloc_scratch_setup(DeviceState* ds) {
 }
 
 u64 ntt_scratch_elems(u64 n, u64 cols) {
-    // columns go through the scratch in groups of at most 256 MB
+    // columns go through the scratch in groups of at most 2 GiB.  Every group
+    // is one launch of each step, and each launch pays about one round of
+    // blocks to fill and drain the GPU (~35 us at 2^20 rows, a tenth of a
+    // 32-column launch), so the groups are made large; the cap bounds the
+    // scratch (a 2^20 x 379 table takes two groups).
     u64 group = cols;
-    u64 max_group = (256ull << 20) / (n * sizeof(u64));
+    u64 max_group = (2048ull << 20) / (n * sizeof(u64));
     if (max_group < 1) max_group = 1;
     if (group > max_group) group = max_group;
     return group * n;
