@@ -216,7 +216,8 @@ int tvh_lde_autotune(size_t num_cols, size_t trace_len, size_t target_len, int* 
 
 /* Phase 1: d_coefficients[c*n + i] = INTT_n(trace_c)_i * trace_offset^-i, the
  * trace's interpolant in coefficient form.  d_coefficients may equal d_trace
- * (in place).  Uses the per-device arena for the NTT's transpose step.
+ * (in place) or must not overlap it.  Uses the per-device arena for the NTT's
+ * transpose step.
  * Replaces compute_trace_coefficients_gpu (randomized_lde_kernel.cuh:127-134). */
 int tvh_trace_coefficients_dev(const uint64_t* d_trace, size_t num_cols, size_t trace_len,
                                uint64_t trace_offset, uint64_t* d_coefficients, void* stream);

@@ -12,13 +12,15 @@ P = 0xFFFFFFFF00000001
 
 @pytest.mark.parametrize("n,cols,toff,inplace", [(1, 2, 1, False), (16, 3, 1, True),
                                                 (64, 5, 3, False), (1024, 4, 11, True),
-                                                (1 << 13, 2, 7, False)])
+                                                (1 << 12, 3, 1, False), (1 << 13, 2, 7, False)])
 def test_trace_coefficients(T, oracle, n, cols, toff, inplace):
     x = oracle.splitmix(70 + n, n * cols)
     d = T.dev(x)
     out = d if inplace else T.empty(n * cols)
     T.lib.trace_coefficients(d, cols, n, toff, out)
     got = T.host(out)
+    if not inplace:  # out of place, the trace is only read
+        assert np.array_equal(T.host(d), np.asarray(x, dtype=np.uint64))
     oinv = oracle.inv(toff)
     for c in range(cols):
         coef = oracle.intt(x[c * n:(c + 1) * n])

@@ -160,8 +160,10 @@ const char* tvh_status_string(int s) {
 const char* tvh_last_error(void) { return g_err; }
 
 // ---------------------------------------------------------------- NTT
-// The batched NTT through the per-device arena (grown on first use of a larger size).
-static int ntt_batched_arena(uint64_t* d, size_t n, size_t cols, bool inv, hipStream_t st) {
+// The batched NTT through the per-device arena (grown on first use of a larger
+// size), from src (null: d itself, in place) into d.
+static int ntt_batched_arena(uint64_t* d, size_t n, size_t cols, bool inv, hipStream_t st,
+                             const uint64_t* src = nullptr) {
     if (!d && cols) return set_error(TVH_ERR_INVALID_ARG, "null data");
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
@@ -170,7 +172,7 @@ static int ntt_batched_arena(uint64_t* d, size_t n, size_t cols, bool inv, hipSt
     u64 need = ntt_scratch_elems(n, cols);
     u64* scratch = (u64*)arena(ds, need * sizeof(u64), st);
     if (!scratch) return TVH_ERR_OOM;
-    return launch_ntt_batched(d, n, cols, inv, scratch, need, st);
+    return launch_ntt_batched(src ? src : d, d, n, cols, inv, scratch, need, st);
 }
 
 int tvh_ntt_forward_batched_dev(uint64_t* d, size_t n, size_t cols, void* s) {
@@ -191,7 +193,7 @@ int tvh_ntt_batched_noalloc_dev(uint64_t* d, size_t n, size_t cols, int inverse,
     // set-up (tables, the scratch's address) must have happened: nothing here allocates
     DeviceState* ds = initialized_device();
     if (!ds) return TVH_ERR_DEVICE;
-    return launch_ntt_batched(d, n, cols, inverse != 0, ds->noalloc_scratch, kNoAllocScratchElems,
+    return launch_ntt_batched(d, d, n, cols, inverse != 0, ds->noalloc_scratch, kNoAllocScratchElems,
                               STREAM(s));
 }
 
@@ -400,10 +402,8 @@ int tvh_trace_coefficients_dev(const uint64_t* trace, size_t cols, size_t n, uin
     hipStream_t st = STREAM(s);
     u64* tables = nullptr;
     if (toff != 1 && !(tables = coset_tables(n, cols, st))) return TVH_ERR_OOM;
-    if (coeffs != trace &&
-        hipMemcpyAsync(coeffs, trace, 8 * n * cols, hipMemcpyDeviceToDevice, st) != hipSuccess)
-        return set_error(TVH_ERR_DEVICE, "coefficient copy");
-    int rc = ntt_batched_arena(coeffs, n, cols, true, st);
+    // the column step reads the trace, the row step writes the coefficients
+    int rc = ntt_batched_arena(coeffs, n, cols, true, st, trace);
     if (rc || toff == 1) return rc;
     return launch_scale_table(coeffs, coeffs, n, cols, gl::inv(toff), nullptr, 0, 0, tables, st);
 }
@@ -735,7 +735,7 @@ static int ntt_host(uint64_t* data, size_t n, bool inv) {
     u64* scratch = hc.alloc(n);
     if (hc.rc) return hc.rc;
     if ((rc = hc.up(d, data, n))) return rc;
-    if ((rc = launch_ntt_batched(d, n, 1, inv, scratch, n, hc.st))) return rc;
+    if ((rc = launch_ntt_batched(d, d, n, 1, inv, scratch, n, hc.st))) return rc;
     if ((rc = hc.down(data, d, n))) return rc;
     return hc.sync();
 }

@@ -404,8 +404,8 @@ u64 ntt_scratch_elems(u64 n, u64 cols) {
     return group * n;
 }
 
-int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, u64* scratch, u64 scratch_elems,
-                       hipStream_t st) {
+int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, u64* scratch,
+                       u64 scratch_elems, hipStream_t st) {
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
     u32 logN = log2u(n);
@@ -427,12 +427,13 @@ int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, u64* scratch, u
     int rc;
     for (u64 c0 = 0; c0 < cols; c0 += group) {
         u64 g = cols - c0 < group ? cols - c0 : group;
-        u64* d = data + c0 * n;
+        const u64* sg = src + c0 * n;  // step 1 reads the group from src
+        u64* d = dst + c0 * n;         // step 2 writes it to dst
         {
             u32 logNT = tile_log_nt(a);
             u64 ntr = g << b;
             u64 blocks = (ntr + (1ull << logNT) - 1) >> logNT;
-            Step1Args A{d, n, scratch, n, logN, a, 1u, ntr, logNT};
+            Step1Args A{sg, n, scratch, n, logN, a, 1u, ntr, logNT};
             const size_t s1_lds =
                 (size_t)fused_pitch(1 << a, 1 << logNT, true) * (1u << logNT) * sizeof(u64);
             if ((rc = launch_k_step1(inverse, LOG_TILE, (u32)blocks, s1_lds, st, A, ds->roots)))

@@ -77,13 +77,14 @@ inline u32 log2u(u64 x) {
 }
 
 // ---- kernels launchers (ntt_kernels.hip) ----
-// Batched NTT in place through a scratch of scratch_elems u64 (at least n;
-// columns go through it in groups of scratch_elems / n).  Launches that share
-// a scratch must be ordered (one stream).
-int launch_ntt_batched(u64* data, u64 n, u64 cols, bool inverse, u64* scratch, u64 scratch_elems,
-                       hipStream_t st);
+// Batched NTT from src to dst (the same buffer, or buffers that do not
+// overlap) through a scratch of scratch_elems u64 (at least n; columns go
+// through it in groups of scratch_elems / n).  Launches that share a scratch
+// must be ordered (one stream).
+int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, u64* scratch,
+                       u64 scratch_elems, hipStream_t st);
 // The scratch launch_ntt_batched uses for (n, cols) from the per-device arena
-// (groups of at most 256 MB).
+// (groups of at most 2 GiB).
 u64 ntt_scratch_elems(u64 n, u64 cols);
 // LDE of `cols` B-field columns (column-major in, column-major out).  rand may
 // be null (k = 0).  truncate selects the XFE evaluation rule.  lift_stride:
