@@ -158,6 +158,7 @@ def run_main(args, tvh, lib, hip_rt):
         w = 3 * 88
     trace = hip_rt.DeviceBuffer(n * w)
     lib.fill_table(trace, n, w, {"main": 1, "aux": 2, "frugal": 3}[args.mode])
+    hip_rt.synchronize()  # filled on the null stream; the steps run on a non-blocking one
     frugal = args.mode == "frugal"
     cw = hip_rt.DeviceBuffer(lib.frugal_workspace_elems(w, n, m, args.chunk) if frugal else m * w)
     tree = hip_rt.DeviceBuffer(5 * (2 * m - 1))
@@ -294,6 +295,7 @@ def run_cosets(args, tvh, lib, hip_rt, reps=3):
     lib.fill_table(tr, n, w, 1)
     rand = hip_rt.DeviceBuffer(w * k)
     lib.fill_uniform(rand, w * k, 98)
+    hip_rt.synchronize()  # filled on the null stream; the commits run on a non-blocking one
     coeffs = hip_rt.DeviceBuffer(n * w)
     work = hip_rt.DeviceBuffer(n * w)
     dig = hip_rt.DeviceBuffer(5 * n)

@@ -79,8 +79,12 @@ int launch_k_step1(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, con
                    const Roots& rt) {
     const dim3 tpb(1u << (lt - 4));
     if (!inv) {
-        if (lt != LOG_TILE) return set_error(TVH_ERR_UNSUPPORTED, "forward step 1 tile %u", lt);
-        hipLaunchKernelGGL((k_step1<false, LOG_TILE>), dim3(blocks), tpb, lds, st, A, rt);
+        if (lt == LOG_TILE - 1)
+            hipLaunchKernelGGL((k_step1<false, LOG_TILE - 1>), dim3(blocks), tpb, lds, st, A, rt);
+        else if (lt == LOG_TILE)
+            hipLaunchKernelGGL((k_step1<false, LOG_TILE>), dim3(blocks), tpb, lds, st, A, rt);
+        else
+            return set_error(TVH_ERR_UNSUPPORTED, "forward step 1 tile %u", lt);
     } else if (lt == 12) {
         hipLaunchKernelGGL((k_step1<true, 12>), dim3(blocks), tpb, lds, st, A, rt);
     } else if (lt == 13) {
@@ -112,6 +116,7 @@ int launch_k_ntt_step2(bool inv, u32 blocks, size_t lds, hipStream_t st, const u
 int set_attributes_step1(int bytes) {
     const void* fns[] = {(const void*)k_step1<true, 14>, (const void*)k_step1<false, 14>,
                          (const void*)k_step1<true, 13>, (const void*)k_step1<true, 12>,
+                         (const void*)k_step1<false, 13>,
                          (const void*)k_step1_multi<true, kBatchColLt>,
                          (const void*)k_ntt_step2<true>, (const void*)k_ntt_step2<false>};
     for (const void* f : fns)
