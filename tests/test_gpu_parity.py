@@ -88,10 +88,12 @@ def test_hash_xfield_rows(T, oracle):
 
 
 # ------------------------------------------------------------------ NTT
-@pytest.mark.parametrize("lg", list(range(0, 17)) + [20])
+# every size up to 2^21: from 2^18 rows the split and the column and row tiles
+# change (2^18: both steps in 2^13 tiles; 2^19 up: the row step in 2^14 tiles)
+@pytest.mark.parametrize("lg", list(range(0, 22)))
 def test_ntt_batched(T, oracle, lg):
     n = 1 << lg
-    cols = 3 if lg < 20 else 1
+    cols = 3 if lg < 17 else (2 if lg < 20 else 1)
     x = oracle.splitmix(900 + lg, n * cols)
     d = T.dev(x)
     T.lib.ntt_forward_batched(d, n, cols)

@@ -774,8 +774,9 @@ int launch_k_step1(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, con
                    const Roots& rt);
 int launch_k_step1_multi(const Step1Args* args, const u32* start, u32 nseg, u32 blocks, size_t lds,
                          hipStream_t st, const Roots& rt);
-int launch_k_ntt_step2(bool inv, u32 blocks, size_t lds, hipStream_t st, const u64* src, u64* dst,
-                       u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv_m, const Roots& rt);
+int launch_k_ntt_step2(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, const u64* src,
+                       u64* dst, u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv_m,
+                       const Roots& rt);
 int set_attributes_step1(int lds_bytes);
 // lde_rows.hip
 int launch_k_lde_rowinv(u32 lt, u32 blocks, size_t lds, hipStream_t st, const LdeArgs& A,

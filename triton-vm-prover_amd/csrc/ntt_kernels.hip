@@ -423,6 +423,11 @@ int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, 
         a = logN - b;
     }
     const u32 col_lt = big ? LOG_TILE - 1 : LOG_TILE;
+    // The row step stores each block's rows transposed, in runs as long as its
+    // row count: 2^13 tiles (512 threads) win while they still hold 8 rows
+    // (N2 <= 2^10: 2^18 x 379 inverse 1.09 against 1.26 ms) and lose when
+    // they hold fewer (2^22 x 64, N2 = 2^12: 6.3 against 4.0 ms).
+    const u32 row_lt = (big && b <= 10) ? LOG_TILE - 1 : LOG_TILE;
     // step 1 writes each group of columns to the scratch, step 2 writes it back
     u64 group = scratch_elems / n;
     if (!scratch || group == 0)
@@ -446,14 +451,14 @@ int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, 
                 return rc;
         }
         {
-            u32 logNT = tile_log_nt(b);
+            u32 logNT = tile_log_nt(b, row_lt);
             u64 nrows = g << a;
             if ((1ull << logNT) > nrows) logNT = log2u(nrows);
             u64 blocks = (nrows + (1ull << logNT) - 1) >> logNT;
             const size_t s2_lds =
                 (size_t)fused_pitch(1 << b, 1 << logNT, true) * (1u << logNT) * sizeof(u64);
-            if ((rc = launch_k_ntt_step2(inverse, (u32)blocks, s2_lds, st, scratch, d, logN, a,
-                                         nrows, logNT, ninv_m, ds->roots)))
+            if ((rc = launch_k_ntt_step2(inverse, row_lt, (u32)blocks, s2_lds, st, scratch, d,
+                                         logN, a, nrows, logNT, ninv_m, ds->roots)))
                 return rc;
         }
     }

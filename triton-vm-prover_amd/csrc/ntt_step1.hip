@@ -63,16 +63,18 @@ struct PlainRowStore {
     }
 };
 
-template <bool INV>
-__global__ void __launch_bounds__(TPB) k_ntt_step2(const u64* __restrict__ src,
-                                                   u64* __restrict__ dst, u32 logN, u32 a,
-                                                   u64 nrows, u32 logNT, u64 ninv_m, Roots rt) {
+template <bool INV, int LT>
+__global__ void __launch_bounds__(1 << (LT - 4)) k_ntt_step2(const u64* __restrict__ src,
+                                                             u64* __restrict__ dst, u32 logN, u32 a,
+                                                             u64 nrows, u32 logNT, u64 ninv_m,
+                                                             Roots rt) {
     extern __shared__ u64 lds[];
     const u32 b = logN - a;
     const u64 s0 = (u64)blockIdx.x << logNT;
     const PlainRowLoad ld{src, s0, nrows, b};
     const PlainRowStore<INV> st{dst, 1ull << logN, s0, nrows, ninv_m, a};
-    fused_dft<INV, TPB, TILE, false, true>(lds, b, 1 << logNT, logNT, rt.r4096[INV], ld, st);
+    fused_dft<INV, (1 << (LT - 4)), (1 << LT), false, true>(lds, b, 1 << logNT, logNT,
+                                                           rt.r4096[INV], ld, st);
 }
 
 int launch_k_step1(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, const Step1Args& A,
@@ -102,14 +104,23 @@ int launch_k_step1_multi(const Step1Args* args, const u32* start, u32 nseg, u32 
     return check_launch();
 }
 
-int launch_k_ntt_step2(bool inv, u32 blocks, size_t lds, hipStream_t st, const u64* src, u64* dst,
-                       u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv_m, const Roots& rt) {
-    if (inv)
-        hipLaunchKernelGGL(k_ntt_step2<true>, dim3(blocks), dim3(TPB), lds, st, src, dst, logN, a,
-                           nrows, logNT, ninv_m, rt);
-    else
-        hipLaunchKernelGGL(k_ntt_step2<false>, dim3(blocks), dim3(TPB), lds, st, src, dst, logN, a,
-                           nrows, logNT, ninv_m, rt);
+int launch_k_ntt_step2(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, const u64* src,
+                       u64* dst, u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv_m,
+                       const Roots& rt) {
+    const dim3 tpb(1u << (lt - 4));
+#define TVH_STEP2(I, L) \
+    hipLaunchKernelGGL((k_ntt_step2<I, L>), dim3(blocks), tpb, lds, st, src, dst, logN, a, nrows, \
+                       logNT, ninv_m, rt)
+    if (lt == LOG_TILE) {
+        if (inv) TVH_STEP2(true, LOG_TILE);
+        else TVH_STEP2(false, LOG_TILE);
+    } else if (lt == LOG_TILE - 1) {
+        if (inv) TVH_STEP2(true, LOG_TILE - 1);
+        else TVH_STEP2(false, LOG_TILE - 1);
+    } else {
+        return set_error(TVH_ERR_UNSUPPORTED, "row step tile %u", lt);
+    }
+#undef TVH_STEP2
     return check_launch();
 }
 
@@ -118,7 +129,8 @@ int set_attributes_step1(int bytes) {
                          (const void*)k_step1<true, 13>, (const void*)k_step1<true, 12>,
                          (const void*)k_step1<false, 13>,
                          (const void*)k_step1_multi<true, kBatchColLt>,
-                         (const void*)k_ntt_step2<true>, (const void*)k_ntt_step2<false>};
+                         (const void*)k_ntt_step2<true, 14>, (const void*)k_ntt_step2<false, 14>,
+                         (const void*)k_ntt_step2<true, 13>, (const void*)k_ntt_step2<false, 13>};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
             return set_error(TVH_ERR_DEVICE, "LDS attribute");
