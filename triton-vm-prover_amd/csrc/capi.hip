@@ -163,7 +163,7 @@ const char* tvh_last_error(void) { return g_err; }
 // The batched NTT through the per-device arena (grown on first use of a larger
 // size), from src (null: d itself, in place) into d.
 static int ntt_batched_arena(uint64_t* d, size_t n, size_t cols, bool inv, hipStream_t st,
-                             const uint64_t* src = nullptr) {
+                             const uint64_t* src = nullptr, const NttScale* scale = nullptr) {
     if (!d && cols) return set_error(TVH_ERR_INVALID_ARG, "null data");
     if (cols == 0) return TVH_OK;
     if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
@@ -172,7 +172,7 @@ static int ntt_batched_arena(uint64_t* d, size_t n, size_t cols, bool inv, hipSt
     u64 need = ntt_scratch_elems(n, cols);
     u64* scratch = (u64*)arena(ds, need * sizeof(u64), st);
     if (!scratch) return TVH_ERR_OOM;
-    return launch_ntt_batched(src ? src : d, d, n, cols, inv, scratch, need, st);
+    return launch_ntt_batched(src ? src : d, d, n, cols, inv, scratch, need, st, scale);
 }
 
 int tvh_ntt_forward_batched_dev(uint64_t* d, size_t n, size_t cols, void* s) {
@@ -422,10 +422,10 @@ int tvh_evaluate_coset_dev(const uint64_t* coeffs, size_t cols, size_t n, const 
     u64* tables = coset_tables(n, cols, st);
     if (!tables) return TVH_ERR_OOM;
     u64 delta = gl::sub(gl::pow(coset_offset, n), gl::pow(toff, n));
-    int rc = launch_scale_table(coeffs, out, n, cols, coset_offset, k ? rand : nullptr, k, delta,
-                                tables, st);
-    if (rc) return rc;
-    return ntt_batched_arena(out, n, cols, false, st);
+    // b_i = (c_i + [i < k] r_i delta) g^i, applied as the NTT's column step
+    // loads the coefficients; its row step writes the coset's values to out
+    const NttScale S{coset_offset, tables, k ? rand : nullptr, k, delta};
+    return ntt_batched_arena(out, n, cols, false, st, coeffs, &S);
 }
 
 int tvh_scatter_digests_strided_dev(const uint64_t* coset_digests, uint64_t* leaves,

@@ -144,18 +144,30 @@ int launch_fill_table(u64* d, u64 rows, u64 cols, u64 seed, u64* rej, hipStream_
 
 u64 fill_table_scratch_elems() { return 1 + kMaxRejects; }
 
-u64 scale_table_elems(u64 n) { return 4096 + (n + 4095) / 4096; }
+u64 scale_table_elems(u64 n) {
+    u64 e = 4096 + (n + 4095) / 4096;  // the two-level table of k_scale_table
+    return e > kNttScaleTableElems ? e : kNttScaleTableElems;  // or the NTT's scaling tables
+}
+
+int launch_pow_table(u64* t, u64 base, u64 count, hipStream_t st) {
+    if (!count) return TVH_OK;
+    if (count > (1ull << 24)) return set_error(TVH_ERR_UNSUPPORTED, "power table of %llu",
+                                               (unsigned long long)count);
+    hipLaunchKernelGGL(k_pow_table, dim3((u32)((count + 255) / 256)), dim3(256), 0, st, t, base,
+                       (u32)count);
+    return check_launch();
+}
 
 int launch_scale_table(const u64* x, u64* y, u64 n, u64 cols, u64 base, const u64* rand, u64 k,
                        u64 delta, u64* tables, hipStream_t st) {
     if (!n || !cols) return TVH_OK;
     if (n > (1ull << 24)) return set_error(TVH_ERR_UNSUPPORTED, "scale table for n > 2^24");
-    const u32 nlo = (u32)(n < 4096 ? n : 4096), nhi = (u32)((n + 4095) / 4096);
     u64* tlo = tables;
     u64* thi = tables + 4096;
-    hipLaunchKernelGGL(k_pow_table, dim3((nlo + 255) / 256), dim3(256), 0, st, tlo, base, nlo);
-    hipLaunchKernelGGL(k_pow_table, dim3((nhi + 255) / 256), dim3(256), 0, st, thi,
-                       gl::pow(base, 4096), nhi);
+    int rc;
+    if ((rc = launch_pow_table(tlo, base, n < 4096 ? n : 4096, st)) ||
+        (rc = launch_pow_table(thi, gl::pow(base, 4096), (n + 4095) / 4096, st)))
+        return rc;
     const u64 total = n * cols;
     const u64 blocks = (total + kTabTpb * kTabPer - 1) / (kTabTpb * kTabPer);
     if (blocks > 0x7FFFFFFFull) return set_error(TVH_ERR_UNSUPPORTED, "scale of %llu elements",

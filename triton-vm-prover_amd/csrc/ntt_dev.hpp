@@ -383,6 +383,48 @@ struct Step1Load {
         }
     }
 };
+// Step1Load with the coset scaling of a forward NTT (NttScale): element
+// i = (n1 << b) + n2 of column c enters as (x + [i < k] rand[c*k + i] *
+// delta) * base^i, base^i = p1[n1] * p2[n2] (p2[n2] is the same for all of a
+// thread's elements).  The p1 entries are loaded with the elements, so all
+// loads are in flight together; two products per element replace a separate
+// read-and-write pass over the table.
+struct Step1ScaleLoad {
+    const u64* src;
+    u64 sstride, s0, ntr;
+    u32 b;
+    NttScaleDev S;  // rand already offset to the launch's first column
+    template <int R>
+    __device__ __forceinline__ void load(int t, int j, int per, u64 (&v)[R]) const {
+        u64 s = s0 + t;
+        if (s < ntr) {
+            const u64 mask = (1ull << b) - 1;
+            const u64* base = src + (s0 >> b) * sstride + (s0 & mask);
+            const u32 o = (u32)(((s >> b) - (s0 >> b)) * sstride + (s & mask) - (s0 & mask));
+            const u32 n2 = (u32)(s & mask);
+            const u64 f2 = S.p2[n2];
+            u64 f1[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                v[r] = base[o + ((u32)(j + r * per) << b)];
+                f1[r] = S.p1[j + r * per];
+            }
+            if (n2 < S.k) {  // i >= n2: only such transforms hold randomized elements
+                const u64 c = s >> b;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const u64 i = ((u64)(j + r * per) << b) + n2;
+                    if (i < S.k) v[r] = gl::add(v[r], gl::mul(S.rand[c * S.k + i], S.delta));
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = gl::mont_mul(gl::mont_mul(v[r], f1[r]), f2);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = 0;
+        }
+    }
+};
 // Output k1 of transform (c, n2), times w_N^(+-n2*k1), to dst[c*dstride + E*N2*k1 + n2].
 template <bool INV>
 struct Step1Store {
@@ -430,6 +472,19 @@ __device__ __forceinline__ void step1_body(const Step1Args& A, u64 blk, const Ro
     const Step1Load ld{A.src, A.sstride, s0, A.ntr, b};
     const Step1Store<INV> st{A.dst, A.dstride, s0, A.ntr, b, A.E, A.logN, rt};
     fused_dft<INV, NTPB, 1 << LT, true>(lds, A.a, 1 << A.logNT, A.logNT, rt.r4096[INV], ld, st);
+}
+
+// Forward step 1 with the input scaling (Step1ScaleLoad).
+template <int LT>
+__device__ __forceinline__ void step1_scaled_body(const Step1Args& A, const NttScaleDev& S,
+                                                  u64 blk, const Roots& rt) {
+    extern __shared__ u64 lds[];
+    constexpr int NTPB = 1 << (LT - 4);
+    const u32 b = A.logN - A.a;
+    const u64 s0 = blk << A.logNT;
+    const Step1ScaleLoad ld{A.src, A.sstride, s0, A.ntr, b, S};
+    const Step1Store<false> st{A.dst, A.dstride, s0, A.ntr, b, A.E, A.logN, rt};
+    fused_dft<false, NTPB, 1 << LT, true>(lds, A.a, 1 << A.logNT, A.logNT, rt.r4096[0], ld, st);
 }
 
 
@@ -774,6 +829,8 @@ int launch_k_step1(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, con
                    const Roots& rt);
 int launch_k_step1_multi(const Step1Args* args, const u32* start, u32 nseg, u32 blocks, size_t lds,
                          hipStream_t st, const Roots& rt);
+int launch_k_step1_scaled(u32 lt, u32 blocks, size_t lds, hipStream_t st, const Step1Args& A,
+                          const NttScaleDev& S, const Roots& rt);
 int launch_k_ntt_step2(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, const u64* src,
                        u64* dst, u32 logN, u32 a, u64 nrows, u32 logNT, u64 ninv_m,
                        const Roots& rt);

@@ -405,8 +405,9 @@ u64 ntt_scratch_elems(u64 n, u64 cols) {
 }
 
 int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, u64* scratch,
-                       u64 scratch_elems, hipStream_t st) {
+                       u64 scratch_elems, hipStream_t st, const NttScale* scale) {
     if (cols == 0) return TVH_OK;
+    if (scale && inverse) return set_error(TVH_ERR_INVALID_ARG, "input scaling of an inverse NTT");
     if (!is_pow2(n)) return set_error(TVH_ERR_NOT_POW2, "NTT size must be a power of 2");
     u32 logN = log2u(n);
     if (logN > kMaxLogN) return set_error(TVH_ERR_UNSUPPORTED, "NTT size > 2^24");
@@ -423,6 +424,16 @@ int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, 
         a = logN - b;
     }
     const u32 col_lt = big ? LOG_TILE - 1 : LOG_TILE;
+    NttScaleDev S{};
+    if (scale) {  // p1[n1] = base^(N2 n1), p2[n2] = base^n2
+        if (!scale->tables) return set_error(TVH_ERR_INVALID_ARG, "no tables for the NTT scaling");
+        S = NttScaleDev{scale->tables, scale->tables + (1ull << a), scale->rand, scale->k,
+                        scale->delta};
+        int rc;
+        if ((rc = launch_pow_table(scale->tables, gl::pow(scale->base, 1ull << b), 1ull << a, st)) ||
+            (rc = launch_pow_table(scale->tables + (1ull << a), scale->base, 1ull << b, st)))
+            return rc;
+    }
     // The row step stores each block's rows transposed, in runs as long as its
     // row count: 2^13 tiles (512 threads) win while they still hold 8 rows
     // (N2 <= 2^10: 2^18 x 379 inverse 1.09 against 1.26 ms) and lose when
@@ -447,8 +458,14 @@ int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, 
             Step1Args A{sg, n, scratch, n, logN, a, 1u, ntr, logNT};
             const size_t s1_lds =
                 (size_t)fused_pitch(1 << a, 1 << logNT, true) * (1u << logNT) * sizeof(u64);
-            if ((rc = launch_k_step1(inverse, col_lt, (u32)blocks, s1_lds, st, A, ds->roots)))
-                return rc;
+            if (scale) {
+                NttScaleDev Sg = S;
+                if (Sg.rand) Sg.rand += c0 * Sg.k;  // the group's first column
+                rc = launch_k_step1_scaled(col_lt, (u32)blocks, s1_lds, st, A, Sg, ds->roots);
+            } else {
+                rc = launch_k_step1(inverse, col_lt, (u32)blocks, s1_lds, st, A, ds->roots);
+            }
+            if (rc) return rc;
         }
         {
             u32 logNT = tile_log_nt(b, row_lt);

@@ -18,6 +18,12 @@ __global__ void __launch_bounds__(1 << (LT - 4), 4) k_step1_multi(const Step1Arg
     step1_body<INV, LT>(A, blockIdx.x - start[seg], rt);
 }
 
+template <int LT>
+__global__ void __launch_bounds__(1 << (LT - 4), 4) k_step1_scaled(Step1Args A, NttScaleDev S,
+                                                                   Roots rt) {
+    step1_scaled_body<LT>(A, S, blockIdx.x, rt);
+}
+
 // Plain NTT row step.  Rows s = (c, k1) of N2 = 2^b elements at src[s*N2 + n2]
 // (the column step's output); DFT over n2 -> k2, times N^-1 when INV; element
 // k2 of row (c, k1) to dst[c*N + k1 + N1*k2].  Fused endpoints: the first pass
@@ -97,6 +103,18 @@ int launch_k_step1(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st, con
     return check_launch();
 }
 
+int launch_k_step1_scaled(u32 lt, u32 blocks, size_t lds, hipStream_t st, const Step1Args& A,
+                          const NttScaleDev& S, const Roots& rt) {
+    const dim3 tpb(1u << (lt - 4));
+    if (lt == LOG_TILE - 1)
+        hipLaunchKernelGGL((k_step1_scaled<LOG_TILE - 1>), dim3(blocks), tpb, lds, st, A, S, rt);
+    else if (lt == LOG_TILE)
+        hipLaunchKernelGGL((k_step1_scaled<LOG_TILE>), dim3(blocks), tpb, lds, st, A, S, rt);
+    else
+        return set_error(TVH_ERR_UNSUPPORTED, "scaled step 1 tile %u", lt);
+    return check_launch();
+}
+
 int launch_k_step1_multi(const Step1Args* args, const u32* start, u32 nseg, u32 blocks, size_t lds,
                          hipStream_t st, const Roots& rt) {
     hipLaunchKernelGGL((k_step1_multi<true, kBatchColLt>), dim3(blocks), dim3(1u << (kBatchColLt - 4)),
@@ -127,7 +145,8 @@ int launch_k_ntt_step2(bool inv, u32 lt, u32 blocks, size_t lds, hipStream_t st,
 int set_attributes_step1(int bytes) {
     const void* fns[] = {(const void*)k_step1<true, 14>, (const void*)k_step1<false, 14>,
                          (const void*)k_step1<true, 13>, (const void*)k_step1<true, 12>,
-                         (const void*)k_step1<false, 13>,
+                         (const void*)k_step1<false, 13>, (const void*)k_step1_scaled<13>,
+                         (const void*)k_step1_scaled<14>,
                          (const void*)k_step1_multi<true, kBatchColLt>,
                          (const void*)k_ntt_step2<true, 14>, (const void*)k_ntt_step2<false, 14>,
                          (const void*)k_ntt_step2<true, 13>, (const void*)k_ntt_step2<false, 13>};

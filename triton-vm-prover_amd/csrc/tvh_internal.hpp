@@ -77,12 +77,32 @@ inline u32 log2u(u64 x) {
 }
 
 // ---- kernels launchers (ntt_kernels.hip) ----
+// Input scaling of a forward batched NTT, applied as its column step loads:
+// element i of column c enters as (x + [i < k] rand[c*k + i] * delta) * base^i.
+// tables: kNttScaleTableElems u64 of device memory for the launch's powers of
+// base; rand is null when k = 0.
+struct NttScale {
+    u64 base;
+    u64* tables;
+    const u64* rand;
+    u64 k, delta;
+};
+constexpr u64 kNttScaleTableElems = 8192;  // N1 + N2 <= 2 * 4096
+// What the column step reads (NttScale with the launch's tables filled in):
+// element i = n1 * N2 + n2 takes base^i = p1[n1] * p2[n2], p1[n1] = base^(N2 n1),
+// p2[n2] = base^n2 (Montgomery form).
+struct NttScaleDev {
+    const u64* p1;
+    const u64* p2;
+    const u64* rand;
+    u64 k, delta;
+};
 // Batched NTT from src to dst (the same buffer, or buffers that do not
 // overlap) through a scratch of scratch_elems u64 (at least n; columns go
 // through it in groups of scratch_elems / n).  Launches that share a scratch
-// must be ordered (one stream).
+// must be ordered (one stream).  scale (forward only) scales the input.
 int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, u64* scratch,
-                       u64 scratch_elems, hipStream_t st);
+                       u64 scratch_elems, hipStream_t st, const NttScale* scale = nullptr);
 // The scratch launch_ntt_batched uses for (n, cols) from the per-device arena
 // (groups of at most 2 GiB).
 u64 ntt_scratch_elems(u64 n, u64 cols);
@@ -210,6 +230,8 @@ int launch_xfe_interleave(const u64* src, u64* dst, u64 cols, u64 len, bool to_l
 u64 scale_table_elems(u64 n);
 int launch_scale_table(const u64* x, u64* y, u64 n, u64 cols, u64 base, const u64* rand, u64 k,
                        u64 delta, u64* tables, hipStream_t st);
+// t[i] = base^i for i < count, Montgomery form
+int launch_pow_table(u64* t, u64 base, u64 count, hipStream_t st);
 // leaves[(j * num_cosets + coset) * 5 + e] = dig[j * 5 + e], j < len
 int launch_scatter_digests(const u64* dig, u64* leaves, u64 len, u64 coset, u64 num_cosets,
                            hipStream_t st);
