@@ -251,7 +251,8 @@ def run_main(args, tvh, lib, hip_rt):
         out["roofline"]["valu"] = {
             "busy_cycles_per_cell": lde_cyc,
             "frac": lde_cyc * cells / (lde_ms / 1000) / (SIMDS * CLOCK_HZ),
-            "note": "the LDE kernels are VALU-issue bound: this is the roofline that binds them"}
+            "note": "the LDE kernels are bound by VALU issue and its dependency latency at 4 "
+                    "waves per SIMD, not by HBM: frac is their share of the chip's peak issue"}
     cyc = profile_value(counters, "valu_busy_cycles_per_permutation")
     tip5 = {"permutations": perms, "permutations_per_s": perms / ((hash_ms + merkle_ms) / 1000),
             "row_hash_permutations_per_s": perms_rows / (hash_ms / 1000)}
