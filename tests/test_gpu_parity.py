@@ -88,9 +88,10 @@ def test_hash_xfield_rows(T, oracle):
 
 
 # ------------------------------------------------------------------ NTT
-# every size up to 2^21: from 2^18 rows the split and the column and row tiles
-# change (2^18: both steps in 2^13 tiles; 2^19 up: the row step in 2^14 tiles)
-@pytest.mark.parametrize("lg", list(range(0, 22)))
+# every size up to 2^24: the split and the column and row tiles change at
+# 2^18 (both steps in 2^13 tiles), 2^19 (the row step back in 2^14 tiles) and
+# 2^23 (both in 2^14 tiles, rows capped at 2^12)
+@pytest.mark.parametrize("lg", list(range(0, 25)))
 def test_ntt_batched(T, oracle, lg):
     n = 1 << lg
     cols = 3 if lg < 17 else (2 if lg < 20 else 1)

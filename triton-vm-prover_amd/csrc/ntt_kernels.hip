@@ -413,11 +413,13 @@ int launch_ntt_batched(const u64* src, u64* dst, u64 n, u64 cols, bool inverse, 
     if (logN > kMaxLogN) return set_error(TVH_ERR_UNSUPPORTED, "NTT size > 2^24");
     DeviceState* ds = current_device();
     if (!ds) return TVH_ERR_DEVICE;
-    // Split N = N1 * N2 and column tile, measured on MI355X: from 2^18 rows the
-    // column step takes N1 = 2^(logN/2 - 1) points in 2^13-element tiles (two
-    // 512-thread blocks per CU), 6-13 % faster than N1 = 2^(logN/2) in 2^14
-    // tiles at 2^18..2^22 (2^20 x 379 forward: 5.43 against 5.80 ms).
-    const bool big = logN >= 18;
+    // Split N = N1 * N2 and column tile, measured on MI355X: for 2^18..2^22 rows
+    // the column step takes N1 = 2^(logN/2 - 1) points in 2^13-element tiles
+    // (two 512-thread blocks per CU), 3-13 % faster than N1 = 2^(logN/2) in
+    // 2^14 tiles (2^20 x 379 forward: 5.43 against 5.80 ms).  Above 2^22 the
+    // row length is capped at 2^12, and 2^13 tiles would hold only 4 or 2
+    // columns' transforms (32 or 16 B segments), so the 2^14 tiles stay.
+    const bool big = logN >= 18 && logN <= 22;
     u32 a = logN / 2 - (big ? 1 : 0), b = logN - a;
     if (b > 12) {
         b = 12;
