@@ -228,7 +228,10 @@ int tvh_trace_coefficients_dev(const uint64_t* d_trace, size_t num_cols, size_t 
  * Called once per coset g_k = fri_offset * w_M^k of an M-point domain, row j
  * of coset k is row j*(M/n) + k of the M-point codeword.  d_tail_scratch is
  * accepted for the reference's signature and not used (may be NULL).  Uses the
- * per-device arena for the NTT's transpose step.
+ * per-device arena for the NTT's transpose step and for the powers of
+ * coset_offset that its column step applies to the coefficients as it loads
+ * them.  d_output may equal d_coefficients (in place, which consumes the
+ * coefficients) or must not overlap it.
  * Replaces evaluate_coset_from_coefficients_gpu (randomized_lde_kernel.cuh:152-163). */
 int tvh_evaluate_coset_dev(const uint64_t* d_coefficients, size_t num_cols, size_t trace_len,
                            const uint64_t* d_rand, size_t rand_len, uint64_t trace_offset,

@@ -53,6 +53,34 @@ def test_evaluate_cosets_equal_the_lde(T, oracle, n, cols, K, k, toff):
         assert np.array_equal(got, want[:, kk::K]), kk
 
 
+@pytest.mark.parametrize("n,cols", [(1 << 10, 3), (1 << 20, 300)])
+def test_evaluate_coset_in_place(T, oracle, n, cols):
+    """d_output == d_coefficients gives the out-of-place values.  At 2^20 x 300
+    the NTT runs in two column groups (256 + 44 columns), each read before it
+    is written; the buffers are compared through their row digests."""
+    k, toff = 7, 1
+    tr, r = T.empty(n * cols), T.empty(k * cols)
+    T.lib.fill_uniform(tr, n * cols, 500 + n)
+    T.lib.fill_uniform(r, k * cols, 501)
+    coeffs = T.empty(n * cols)
+    T.lib.trace_coefficients(tr, cols, n, toff, coeffs)
+    del tr
+    g = oracle.mul(7, oracle.root_of_unity(3))
+
+    def digests(buf):
+        d = T.empty(5 * n)
+        T.lib.hash_bfield_rows(buf, n, cols, d)
+        return T.host(d)
+
+    before = digests(coeffs)
+    out = T.empty(n * cols)
+    T.lib.evaluate_coset(coeffs, cols, n, r, k, toff, g, out)
+    T.lib.evaluate_coset(coeffs, cols, n, r, k, toff, g, coeffs)
+    want = digests(out)
+    assert np.array_equal(digests(coeffs), want)
+    assert not np.array_equal(want, before)
+
+
 def _two_phase_commit(T, dt, w, n, K, dr, k, toff, goff):
     """The frugal caller's sequence (gpu_stark.cu:2356-2420): coefficients once,
     then per coset: evaluate, hash rows, scatter the digests at stride K; then
