@@ -32,6 +32,7 @@ def test_trace_coefficients(T, oracle, n, cols, toff, inplace):
 
 @pytest.mark.parametrize("n,cols,K,k,toff", [(16, 3, 4, 0, 1), (64, 2, 8, 5, 1), (32, 4, 2, 32, 3),
                                              (256, 3, 8, 98, 1), (8, 2, 16, 3, 7),
+                                             (1, 2, 4, 1, 1), (2, 3, 8, 2, 3),
                                              (1 << 12, 2, 4, 1 << 12, 3), (1 << 18, 2, 4, 98, 1)])
 def test_evaluate_cosets_equal_the_lde(T, oracle, n, cols, K, k, toff):
     """Coset kk of the M = K n point domain 7 <w_M> is 7 w_M^kk <w_n>: its row j is
