@@ -252,7 +252,9 @@ def run_main(args, tvh, lib, hip_rt):
             "busy_cycles_per_cell": lde_cyc,
             "frac": lde_cyc * cells / (lde_ms / 1000) / (SIMDS * CLOCK_HZ),
             "note": "the LDE kernels are bound by VALU issue and its dependency latency at 4 "
-                    "waves per SIMD, not by HBM: frac is their share of the chip's peak issue"}
+                    "waves per SIMD, not by HBM: frac is their share of the chip's peak issue, "
+                    "busy cycles from counted instructions at 2/4/8 SIMD cycles per 32-bit/"
+                    "f64/64-bit-integer wave op (a model; an upper estimate)"}
     cyc = profile_value(counters, "valu_busy_cycles_per_permutation")
     tip5 = {"permutations": perms, "permutations_per_s": perms / ((hash_ms + merkle_ms) / 1000),
             "row_hash_permutations_per_s": perms_rows / (hash_ms / 1000)}
@@ -260,7 +262,9 @@ def run_main(args, tvh, lib, hip_rt):
         busy = cyc * perms_rows / (hash_ms / 1000)  # SIMD cycles/s the kernel keeps busy
         tip5.update({"valu_frac": busy / (SIMDS * CLOCK_HZ),
                      "valu_busy_cycles_per_permutation": cyc,
-                     "peak_simd_cycles_per_s": SIMDS * CLOCK_HZ})
+                     "peak_simd_cycles_per_s": SIMDS * CLOCK_HZ,
+                     "valu_note": "busy cycles from counted instructions at 2/4/8 SIMD cycles per "
+                                  "32-bit/f64/64-bit-integer wave op (a model; an upper estimate)"})
     out["tip5"] = tip5
     if frugal:
         out["phases_ms"] = {"commit_streamed": lde_ms}
