@@ -82,6 +82,30 @@ def test_evaluate_coset_in_place(T, oracle, n, cols):
     assert not np.array_equal(want, before)
 
 
+@pytest.mark.parametrize("lg", [22, 23])
+def test_evaluate_cosets_equal_the_lde_large(T, oracle, lg):
+    """At 2^22 and 2^23 rows (the batched NTT's large splits: 2^13 column tiles at
+    2^22, 2^14 tiles and capped rows at 2^23) every coset of the 4N-point domain,
+    98 randomizers, equals the matching rows of the LDE path, whose kernels are
+    separate and oracle-checked."""
+    n, cols, K, k, toff = 1 << lg, 2, 4, 98, 1
+    m = K * n
+    tr, r = T.empty(n * cols), T.empty(k * cols)
+    T.lib.fill_uniform(tr, n * cols, 700 + lg)
+    T.lib.fill_uniform(r, k * cols, 701)
+    cw = T.empty(m * cols)
+    T.lib.lde_batch(tr, cols, n, r, k, toff, 7, m, cw)
+    want = T.host(cw).reshape(cols, m)
+    del cw
+    coeffs, out = T.empty(n * cols), T.empty(n * cols)
+    T.lib.trace_coefficients(tr, cols, n, toff, coeffs)
+    wm = oracle.root_of_unity(lg + 2)
+    for kk in range(K):
+        g = oracle.mul(7, oracle.pow(wm, kk))
+        T.lib.evaluate_coset(coeffs, cols, n, r, k, toff, g, out)
+        assert np.array_equal(T.host(out).reshape(cols, n), want[:, kk::K]), kk
+
+
 def _two_phase_commit(T, dt, w, n, K, dr, k, toff, goff):
     """The frugal caller's sequence (gpu_stark.cu:2356-2420): coefficients once,
     then per coset: evaluate, hash rows, scatter the digests at stride K; then
