@@ -173,6 +173,39 @@ __device__ __forceinline__ void permutation_m(u64 s[STATE], const uint8_t* lut) 
     for (int r = 0; r < ROUNDS; r++) round_m(s, r, lut);
 }
 
+// The permutation spread over a group of 16 lanes (16-aligned within the wave),
+// for latency-bound work such as the top levels of a Merkle tree: lane w of the
+// group holds state word w (Montgomery form).  The S-box acts lane-wise; the
+// circulant MDS y_w = sum_c M[(w - c) & 15] s_c gathers the group's words with
+// lane shuffles and sums each 32-bit half in f64, exact because sum_c M_c <
+// 2^20 keeps both sums below 2^52.  Every lane of the group must take part.
+__constant__ double kMdsColumnF64[STATE] = {61402, 1108,  28750, 33823, 7454,  43244, 53865, 12034,
+                                            56951, 27521, 41351, 40901, 12021, 59689, 26798, 17845};
+static_assert(tip5c::kMdsFirstColumn[0] == 61402 && tip5c::kMdsFirstColumn[15] == 17845,
+              "MDS column");
+
+__device__ __forceinline__ u64 permutation_group16(u64 s, u32 w, const uint8_t* lut) {
+    double m[STATE];
+#pragma unroll
+    for (int c = 0; c < STATE; c++) m[c] = kMdsColumnF64[(w - (u32)c) & 15];
+#pragma unroll 1
+    for (int r = 0; r < ROUNDS; r++) {
+        s = w < 4 ? split_and_lookup_m(s, lut) : pow7_m(s);
+        const u32 lo = (u32)s, hi = (u32)(s >> 32);
+        double l0 = 0.0, l1 = 0.0, h0 = 0.0, h1 = 0.0;  // two chains per half
+#pragma unroll
+        for (int c = 0; c < STATE; c += 2) {
+            l0 = __fma_rn(m[c], (double)__shfl(lo, c, 16), l0);
+            h0 = __fma_rn(m[c], (double)__shfl(hi, c, 16), h0);
+            l1 = __fma_rn(m[c + 1], (double)__shfl(lo, c + 1, 16), l1);
+            h1 = __fma_rn(m[c + 1], (double)__shfl(hi, c + 1, 16), h1);
+        }
+        const double* bias = kRcBias.v + (r * STATE + w) * 2;
+        s = gl::lin52(f2u52(l0 + l1, bias[0]), f2u52(h0 + h1, bias[1]));
+    }
+    return s;
+}
+
 // Montgomery form <-> canonical
 __device__ __forceinline__ u64 to_m(u64 x) { return gl::to_mont(x); }
 __device__ __forceinline__ u64 from_m(u64 y) { return gl::montyred(y, 0); }

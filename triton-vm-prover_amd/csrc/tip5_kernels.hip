@@ -11,8 +11,9 @@
 // Node (level l, index i) of an n-leaf tree lives at base(l) + i with
 //   level order (merkle_kernel.cu:60-102): base(l) = 2n - (2n >> l)
 //   heap        (merkle_tree.cpp:12-33):   base(l) = n >> l      (1-indexed)
-// One launch builds up to 9 levels: a block loads 2T children, hashes T
-// parents, keeps them in LDS and continues to the block's subtree root.
+// Levels of more than 2^16 parents take a launch each, one hash per thread.
+// Above them a block hashes up to 16 parents on 16 lanes each, keeps them in
+// LDS and continues to their subtree root (up to 5 levels per launch).
 #include "tvh_internal.hpp"
 
 namespace tvh {
@@ -251,12 +252,45 @@ __device__ __forceinline__ void merkle_body(const MerkleArgs& A, u64 blk, const 
     }
 }
 
+// The same levels with each hash on a group of 16 lanes (tip5::permutation_group16):
+// a 256-thread block holds 16 groups, hashes T <= 16 parents of its first level
+// and continues in LDS to their subtree root.  Group g, lane w: state word w of
+// parent g is child word w (the two children are adjacent: words 0..9), then
+// capacity ones.
+__device__ __forceinline__ void merkle_body_group16(const MerkleArgs& A, u64 blk,
+                                                    const uint8_t* lut, u64* buf) {
+    const u32 g = threadIdx.x >> 4, w = threadIdx.x & 15;
+    u32 width = A.T;
+    u64 s = tip5::kOneM;
+    if (g < width && w < 10) s = tip5::to_m(A.src[10 * (blk * width + g) + w]);
+    s = tip5::permutation_group16(s, w, lut);
+    if (g < width && w < tip5::DIGEST) {
+        const u64 d = tip5::from_m(s);
+        A.dst[0][5 * (blk * width + g) + w] = d;
+        buf[5 * g + w] = d;
+    }
+    for (u32 j = 1; j < A.levels; j++) {
+        __syncthreads();  // the previous level's digests are in buf
+        width >>= 1;
+        s = tip5::kOneM;
+        if (g < width && w < 10) s = tip5::to_m(buf[10 * g + w]);
+        __syncthreads();  // every read of buf precedes this level's writes
+        s = tip5::permutation_group16(s, w, lut);
+        if (g < width && w < tip5::DIGEST) {
+            const u64 d = tip5::from_m(s);
+            A.dst[j][5 * (blk * width + g) + w] = d;
+            buf[5 * g + w] = d;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_merkle_subtrees(MerkleArgs A) {
     __shared__ uint8_t lut[256];
     __shared__ u64 buf[256 * 5];
     tip5::load_lut(lut);
     __syncthreads();
-    merkle_body(A, blockIdx.x, lut, buf);
+    if (A.group16) merkle_body_group16(A, blockIdx.x, lut, buf);
+    else merkle_body(A, blockIdx.x, lut, buf);
 }
 
 __global__ void __launch_bounds__(256) k_merkle_subtrees_multi(const MerkleArgs* __restrict__ args,
@@ -268,7 +302,8 @@ __global__ void __launch_bounds__(256) k_merkle_subtrees_multi(const MerkleArgs*
     __syncthreads();
     u32 seg = find_seg(start, nseg, blockIdx.x);
     const MerkleArgs& A = args[seg];  // read in place: a private copy would spill dst[]
-    merkle_body(A, blockIdx.x - start[seg], lut, buf);
+    if (A.group16) merkle_body_group16(A, blockIdx.x - start[seg], lut, buf);
+    else merkle_body(A, blockIdx.x - start[seg], lut, buf);
 }
 
 // Gather: out[q] = node with heap index idx[q] (1 <= idx < 2n) of a tree in
@@ -417,16 +452,23 @@ int merkle_groups(u64* tree, u64 n, int layout, std::vector<MerkleGroup>& groups
     while (lvl < logn) {
         u64 nchild = n >> lvl;
         u64 parents = nchild / 2;
-        u32 T = parents >= 256 ? 256 : (u32)parents;  // parents is a power of two
+        // Levels of more than 2^15 parents take a launch each with one hash
+        // per thread (continuing inside a block would halve the active threads
+        // per level).  Above them one hash on one lane is a latency of ~21 us
+        // per level, so each hash runs on 16 lanes (permutation_group16, ~6 us):
+        // a 256-thread block hashes up to 16 parents and, once all blocks fit
+        // on the GPU at once (<= 2^14 parents), continues to their subtree root,
+        // up to 5 levels per launch.  Measured at 2^22 leaves: levels 2^16..1
+        // in 387 us with one lane per hash, 152 us in this form.
+        const bool group16 = parents <= (1u << 15);
+        u32 T = group16 ? (parents >= 16 ? 16 : (u32)parents)
+                        : (parents >= 256 ? 256 : (u32)parents);  // parents is a power of two
         u32 maxlev = 0;
-        while ((1u << maxlev) <= T) maxlev++;           // T=256 -> 9 levels
-        // Large levels get a launch each: continuing inside a block halves the
-        // active threads per level, so a 9-level group keeps the GPU ~22 % busy.
-        // Only the top levels (<= 2^16 parents), which cannot fill the GPU
-        // anyway, are built as in-block subtrees.
-        if (parents > (1u << 16)) maxlev = 1;
+        while ((1u << maxlev) <= T) maxlev++;  // T = 16 -> 5 levels
+        if (!group16 || parents > (1u << 14)) maxlev = 1;
         u32 levels = logn - lvl < maxlev ? logn - lvl : maxlev;
         MerkleGroup g;
+        g.args.group16 = group16 ? 1u : 0u;
         g.args.src = tree + 5 * merkle_level_base(n, lvl, layout);
         for (u32 j = 0; j < 9; j++)
             g.args.dst[j] = j < levels ? tree + 5 * merkle_level_base(n, lvl + j + 1, layout) : nullptr;
@@ -444,7 +486,8 @@ int launch_merkle_levels(u64* tree, u64 n, int layout, hipStream_t st) {
     std::vector<MerkleGroup> groups;
     merkle_groups(tree, n, layout, groups);
     for (const MerkleGroup& g : groups) {
-        hipLaunchKernelGGL(k_merkle_subtrees, dim3(g.blocks), dim3(g.args.T), 0, st, g.args);
+        hipLaunchKernelGGL(k_merkle_subtrees, dim3(g.blocks), dim3(g.args.group16 ? 256 : g.args.T),
+                           0, st, g.args);
         int rc = check_launch();
         if (rc) return rc;
     }

@@ -253,7 +253,9 @@ struct MerkleArgs {
     const u64* src;  // children level
     u64* dst[9];     // parents of the next `levels` levels
     u32 levels;
-    u32 T;           // active threads per block (parents per block at the first level)
+    u32 T;           // parents per block at the first level (one thread each, or one
+                     // 16-lane group each when group16)
+    u32 group16;     // 1: each hash runs on 16 lanes (permutation_group16), 256-thread blocks
 };
 struct MerkleGroup {
     MerkleArgs args;
