@@ -179,15 +179,20 @@ __device__ __forceinline__ void permutation_m(u64 s[STATE], const uint8_t* lut) 
 // circulant MDS y_w = sum_c M[(w - c) & 15] s_c gathers the group's words with
 // lane shuffles and sums each 32-bit half in f64, exact because sum_c M_c <
 // 2^20 keeps both sums below 2^52.  Every lane of the group must take part.
-__constant__ double kMdsColumnF64[STATE] = {61402, 1108,  28750, 33823, 7454,  43244, 53865, 12034,
-                                            56951, 27521, 41351, 40901, 12021, 59689, 26798, 17845};
-static_assert(tip5c::kMdsFirstColumn[0] == 61402 && tip5c::kMdsFirstColumn[15] == 17845,
-              "MDS column");
+struct MdsColumnF64 {
+    double v[STATE];
+};
+constexpr MdsColumnF64 mds_column_f64() {
+    MdsColumnF64 t{};
+    for (int i = 0; i < STATE; i++) t.v[i] = (double)tip5c::kMdsFirstColumn[i];
+    return t;
+}
+__constant__ MdsColumnF64 kMdsColumnF64 = mds_column_f64();
 
 __device__ __forceinline__ u64 permutation_group16(u64 s, u32 w, const uint8_t* lut) {
     double m[STATE];
 #pragma unroll
-    for (int c = 0; c < STATE; c++) m[c] = kMdsColumnF64[(w - (u32)c) & 15];
+    for (int c = 0; c < STATE; c++) m[c] = kMdsColumnF64.v[(w - (u32)c) & 15];
 #pragma unroll 1
     for (int r = 0; r < ROUNDS; r++) {
         s = w < 4 ? split_and_lookup_m(s, lut) : pow7_m(s);
