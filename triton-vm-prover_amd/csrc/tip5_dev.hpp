@@ -51,9 +51,8 @@ constexpr RoundConstantsM mont_round_constants() {
 // 1 in Montgomery form (2^64 mod p)
 constexpr u64 kOneM = gl::EPS;
 
-// First column of the circulant MDS matrix: SHA-256("Tip5") as 16 LE u16:
-// 61402 1108 28750 33823 7454 43244 53865 12034 56951 27521 41351 40901 12021
-// 59689 26798 17845 (folded into kMdsP / kMdsQ below).
+// First column of the circulant MDS matrix: tip5c::kMdsFirstColumn (SHA-256
+// of "Tip5" as 16 LE u16), folded into kMdsP / kMdsQ below.
 
 // Fill a 256-byte LDS table with the S-box ((x+1)^3 mod 257) - 1.
 __device__ __forceinline__ void load_lut(uint8_t* lut) {
@@ -113,12 +112,17 @@ __constant__ RcBias kRcBias = rc_bias_table();
 // (s_lo - s_hi) with kMdsQ = (m_lo - m_hi)/2 give A and B with y_lo = A + B,
 // y_hi = A - B.  Every partial sum is a multiple of 1/2 below 2^52 in
 // magnitude, so all f64 arithmetic is exact (128 FMAs per half instead of 256).
-constexpr double kMdsP[8] = {(61402 + 56951) / 2.0, (1108 + 27521) / 2.0, (28750 + 41351) / 2.0,
-                             (33823 + 40901) / 2.0, (7454 + 12021) / 2.0, (43244 + 59689) / 2.0,
-                             (53865 + 26798) / 2.0, (12034 + 17845) / 2.0};
-constexpr double kMdsQ[8] = {(61402 - 56951) / 2.0, (1108 - 27521) / 2.0, (28750 - 41351) / 2.0,
-                             (33823 - 40901) / 2.0, (7454 - 12021) / 2.0, (43244 - 59689) / 2.0,
-                             (53865 - 26798) / 2.0, (12034 - 17845) / 2.0};
+struct MdsHalves {
+    double v[8];
+};
+constexpr MdsHalves mds_halves(int sign) {
+    MdsHalves t{};
+    for (int i = 0; i < 8; i++)
+        t.v[i] = ((double)tip5c::kMdsFirstColumn[i] + sign * (double)tip5c::kMdsFirstColumn[i + 8]) / 2.0;
+    return t;
+}
+constexpr MdsHalves kMdsP = mds_halves(1);
+constexpr MdsHalves kMdsQ = mds_halves(-1);
 
 __device__ __forceinline__ void mds_half(const double x[STATE], double y[STATE]) {
     double sp[8], sm[8];
@@ -132,8 +136,8 @@ __device__ __forceinline__ void mds_half(const double x[STATE], double y[STATE])
         double A = 0.0, B = 0.0;
 #pragma unroll
         for (int c = 0; c < 8; c++) {
-            A = __fma_rn(kMdsP[(r - c) & 7], sp[c], A);
-            B = __fma_rn(r >= c ? kMdsQ[r - c] : -kMdsQ[r - c + 8], sm[c], B);
+            A = __fma_rn(kMdsP.v[(r - c) & 7], sp[c], A);
+            B = __fma_rn(r >= c ? kMdsQ.v[r - c] : -kMdsQ.v[r - c + 8], sm[c], B);
         }
         y[r] = A + B;
         y[r + 8] = A - B;
